@@ -1,0 +1,83 @@
+/*
+ * sapporo_amd.h -- engine-level C-ABI of libsapporo_amd.so.
+ *
+ * Handle-based form of the reference's host class `sapporo`
+ * (lib/include/sapporohostclass.h:121-176): one handle per engine, so one
+ * process can hold several systems (e.g. the mixed and the all-FP64 kernel
+ * side by side).  The g6 and 6th-order ABIs are thin layers over it.
+ * Arrays are row-major doubles (n x 3 for vectors); all pointers are plain
+ * host pointers owned by the caller.  Returns: 0 on success, -1 on error
+ * (message in sapamd_last_error) unless stated.
+ *
+ * order: 1 = FOURTH (Hermite-4), 2 = SIXTH (Hermite-6)  (defines.h:30)
+ * precision: 1 = DOUBLESINGLE (mixed), 2 = DOUBLE (all FP64) (defines.h:31)
+ */
+#ifndef SAPPORO2_AMD_H
+#define SAPPORO2_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sapamd_engine sapamd_engine;
+
+/* New / delete an engine handle (no reference counterpart: the reference
+ * holds one global instance, sapporoG6lib.cpp:3). */
+sapamd_engine *sapamd_create(void);
+void sapamd_destroy(sapamd_engine *e);
+
+/* sapporohostclass.h:125  open(kernelFile, devices, nprocs, order, precision). */
+int sapamd_open(sapamd_engine *e, int device, int order, int precision);
+/* sapporohostclass.h:127 */
+int sapamd_close(sapamd_engine *e);
+/* sapporohostclass.h:128  i-particles per call (16384). */
+int sapamd_get_n_pipes(sapamd_engine *e);
+/* sapporohostclass.h:129-130 */
+int sapamd_set_time(sapamd_engine *e, double t);
+int sapamd_set_no_time(sapamd_engine *e);
+/* sapporohostclass.h:132-139  set_j_particle(address, index, tj, dtj, mass,
+ * k18, j6, a2, v, x, snp, crk, eps): acc and jrk are the true derivatives
+ * (the g6 layer scales a/2 and j/6 before calling). snp/crk may be NULL. */
+int sapamd_set_j_particle(sapamd_engine *e, int address, int id, double tj, double dtj,
+                          double mass, const double acc[3], const double jrk[3],
+                          const double vel[3], const double pos[3], const double snp[3],
+                          const double crk[3], double eps);
+/* sapporohostclass.h:141-147  startGravCalc; xi, vi, ai are ni x 3; ai may be
+ * NULL for FOURTH; h2 may be NULL (no neighbour lists). */
+int sapamd_start_grav_calc(sapamd_engine *e, int nj, int ni, const int *id, const double *xi,
+                           const double *vi, const double *ai, double eps2, const double *h2);
+/* sapporohostclass.h:149-156  getGravResults; any output may be NULL;
+ * ngb != 0 requests nnb / dsmin (written only for a neighbour below 1e10). */
+int sapamd_get_grav_results(sapamd_engine *e, int nj, int ni, double *acc, double *jrk,
+                            double *snp, double *crk, double *pot, int *nnb, double *dsmin,
+                            int ngb);
+/* sapporohostclass.h:170-171  returns 1 on overflow (a count >= 256). */
+int sapamd_read_ngb_list(sapamd_engine *e);
+/* sapporohostclass.h:172-176  returns 1 when the count >= maxlen. */
+int sapamd_get_ngb_list(sapamd_engine *e, int ipipe, int maxlen, int *n, int *list);
+/* sapporohostclass.h:158 */
+int sapamd_force_prediction(sapamd_engine *e, int nj);
+/* sapporohostclass.h:159-162 */
+int sapamd_retrieve_predicted_j_particle(sapamd_engine *e, int addr, double *mass, double *id,
+                                         double *eps2, double *pos, double *vel, double *acc);
+/* sapporohostclass.h:164-168 */
+int sapamd_retrieve_j_particle_state(sapamd_engine *e, int addr, double *mass, double *id,
+                                     double *eps2, double *pos, double *vel, double *acc,
+                                     double *jrk, double *ppos, double *pvel, double *pacc);
+
+/* Launch shape and instrumentation (no reference counterpart; the reference
+ * fixes its shape in sapporohostclass.cpp:1499-1561). */
+int sapamd_set_force_override(sapamd_engine *e, int P, int S);
+int sapamd_set_timing(sapamd_engine *e, int on);
+float sapamd_last_force_ms(sapamd_engine *e);
+float sapamd_last_predict_ms(sapamd_engine *e);
+int sapamd_last_config(sapamd_engine *e, int *P, int *S, int *jchunk);
+int sapamd_num_cus(sapamd_engine *e);
+int sapamd_synchronize(sapamd_engine *e);
+const char *sapamd_last_error(sapamd_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAPPORO2_AMD_H */

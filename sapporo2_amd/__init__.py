@@ -1,0 +1,299 @@
+"""sapporo2_amd -- direct-summation N-body gravity (Hermite-4/6) on one AMD MI355X.
+
+The product is the pair of C-ABI shared libraries built from ``sapporo2_amd/csrc``:
+
+* ``libsapporo_amd.so``     -- engine handle API (``include/sapporo_amd.h``) and the
+                               GRAPE-6 ``g6_*`` ABI (``include/g6lib.h``)
+* ``libsapporo6th_amd.so``  -- the 6th-order Hermite ABI (``include/sapporo6thlib.h``)
+
+This module is a thin ctypes mirror of the reference's host class ``sapporo``
+(``lib/include/sapporohostclass.h:121-176``) used by the tests and the benchmark.  It
+never falls back to a CPU path: if the HIP library is missing it raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Engine", "lib", "lib6th", "FOURTH", "SIXTH", "DOUBLESINGLE", "DOUBLE",
+           "NPIPES", "NGB_CAP", "LIB_PATH", "LIB6_PATH", "build"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsapporo_amd.so")
+LIB6_PATH = os.path.join(HERE, "libsapporo6th_amd.so")
+
+GRAPE5, FOURTH, SIXTH = 0, 1, 2          # reference defines.h:30
+FLOAT, DOUBLESINGLE, DOUBLE = 0, 1, 2    # reference defines.h:31
+MIXED = DOUBLESINGLE
+NPIPES = 16384                           # reference defines.h:14
+NGB_CAP = 256                            # reference defines.h:9
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+_LIB = None
+_LIB6 = None
+
+
+def build(verbose=False):
+    """Compile the HIP libraries in-tree (hipcc, gfx950)."""
+    import subprocess
+    cmd = ["make", "-s", "-C", os.path.join(HERE, "csrc"), "-j4"]
+    subprocess.check_call(cmd, stdout=None if verbose else subprocess.DEVNULL)
+
+
+# Symbols each library must export (checked by the ABI tests).
+ENGINE_SYMBOLS = [
+    "sapamd_create", "sapamd_destroy", "sapamd_open", "sapamd_close", "sapamd_get_n_pipes",
+    "sapamd_set_time", "sapamd_set_no_time", "sapamd_set_j_particle", "sapamd_start_grav_calc",
+    "sapamd_get_grav_results", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
+    "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
+    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_timing",
+    "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_num_cus",
+    "sapamd_synchronize", "sapamd_last_error",
+]
+
+
+def lib():
+    """The HIP engine library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.sapamd_create.restype = _vp
+        L.sapamd_destroy.argtypes = [_vp]
+        L.sapamd_open.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.sapamd_close.argtypes = [_vp]
+        L.sapamd_get_n_pipes.argtypes = [_vp]
+        L.sapamd_set_time.argtypes = [_vp, ctypes.c_double]
+        L.sapamd_set_no_time.argtypes = [_vp]
+        L.sapamd_set_j_particle.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp,
+                                            _dp, _dp, ctypes.c_double]
+        L.sapamd_start_grav_calc.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp,
+                                             ctypes.c_double, _dp]
+        L.sapamd_get_grav_results.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                              _dp, _ip, _dp, ctypes.c_int]
+        L.sapamd_read_ngb_list.argtypes = [_vp]
+        L.sapamd_get_ngb_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip]
+        L.sapamd_force_prediction.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_retrieve_predicted_j_particle.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                                           _dp, _dp]
+        L.sapamd_retrieve_j_particle_state.argtypes = [_vp, ctypes.c_int] + [_dp] * 10
+        L.sapamd_set_force_override.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.sapamd_set_timing.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_last_force_ms.argtypes = [_vp]
+        L.sapamd_last_force_ms.restype = ctypes.c_float
+        L.sapamd_last_predict_ms.argtypes = [_vp]
+        L.sapamd_last_predict_ms.restype = ctypes.c_float
+        L.sapamd_last_config.argtypes = [_vp, _ip, _ip, _ip]
+        L.sapamd_num_cus.argtypes = [_vp]
+        L.sapamd_synchronize.argtypes = [_vp]
+        L.sapamd_last_error.argtypes = [_vp]
+        L.sapamd_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def lib6th():
+    """The 6th-order ABI library (a separate .so, as in the reference)."""
+    global _LIB6
+    if _LIB6 is None:
+        if not os.path.exists(LIB6_PATH):
+            raise RuntimeError(f"{LIB6_PATH} is missing: build it first")
+        _LIB6 = ctypes.CDLL(LIB6_PATH)
+    return _LIB6
+
+
+def _d(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _pd(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+class Engine:
+    """One GPU engine (handle over ``sapamd_*``).  Methods mirror the reference's
+    ``class sapporo``; arrays are numpy (n, 3) float64."""
+
+    def __init__(self, order=FOURTH, precision=DOUBLESINGLE, device=0):
+        L = lib()
+        self._L = L
+        self._h = L.sapamd_create()
+        if not self._h:
+            raise MemoryError("sapamd_create failed")
+        self.order = order
+        self.precision = precision
+        self._check(L.sapamd_open(self._h, device, order, precision), "open")
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self._L.sapamd_last_error(self._h)
+            raise RuntimeError(f"sapporo2_amd {what} failed: {msg.decode() if msg else ''}")
+        return rc
+
+    def close(self):
+        if self._h:
+            self._L.sapamd_close(self._h)
+            self._L.sapamd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def n_pipes(self):
+        return self._L.sapamd_get_n_pipes(self._h)
+
+    def set_time(self, t):
+        return self._check(self._L.sapamd_set_time(self._h, float(t)), "set_time")
+
+    def set_no_time(self):
+        return self._check(self._L.sapamd_set_no_time(self._h), "set_no_time")
+
+    def set_j_particle(self, address, id, tj, dtj, mass, acc, jrk, vel, pos, snp=None, crk=None,
+                       eps=0.0):
+        a = [_d(v, (3,)) for v in (acc, jrk, vel, pos, snp, crk)]
+        return self._check(self._L.sapamd_set_j_particle(
+            self._h, int(address), int(id), float(tj), float(dtj), float(mass),
+            *[_pd(v) for v in a], float(eps)), "set_j_particle")
+
+    def set_j_particles(self, addresses, ids, mass, pos, vel, acc=None, jrk=None, snp=None,
+                        crk=None, tj=0.0, eps=0.0):
+        """Vector convenience: one set_j_particle call per particle."""
+        n = len(addresses)
+        pos = _d(pos, (n, 3)); vel = _d(vel, (n, 3))
+        z = np.zeros((n, 3))
+        acc = z if acc is None else _d(acc, (n, 3))
+        jrk = z if jrk is None else _d(jrk, (n, 3))
+        snp = None if snp is None else _d(snp, (n, 3))
+        crk = None if crk is None else _d(crk, (n, 3))
+        tj = np.broadcast_to(np.asarray(tj, np.float64), (n,))
+        mass = np.asarray(mass, np.float64)
+        f = self._L.sapamd_set_j_particle
+        h = self._h
+        for k in range(n):
+            rc = f(h, int(addresses[k]), int(ids[k]), float(tj[k]), 0.0, float(mass[k]),
+                   _pd(acc[k]), _pd(jrk[k]), _pd(vel[k]), _pd(pos[k]),
+                   None if snp is None else _pd(snp[k]), None if crk is None else _pd(crk[k]),
+                   float(eps))
+            if rc < 0:
+                self._check(rc, "set_j_particle")
+
+    def start_grav_calc(self, nj, ids, xi, vi, eps2, h2=None, ai=None):
+        ni = len(ids)
+        self._keep = (np.ascontiguousarray(ids, np.int32), _d(xi, (ni, 3)), _d(vi, (ni, 3)),
+                      None if ai is None else _d(ai, (ni, 3)), None if h2 is None else _d(h2, (ni,)))
+        ids_, xi_, vi_, ai_, h2_ = self._keep
+        return self._check(self._L.sapamd_start_grav_calc(
+            self._h, int(nj), ni, _pi(ids_), _pd(xi_), _pd(vi_), _pd(ai_), float(eps2), _pd(h2_)),
+            "start_grav_calc")
+
+    def get_grav_results(self, nj, ni, ngb=True):
+        """Returns dict acc, jrk, snp (SIXTH), pot, nnb, dsmin (NaN where not written)."""
+        acc = np.zeros((ni, 3)); jrk = np.zeros((ni, 3)); pot = np.zeros(ni)
+        snp = np.zeros((ni, 3)) if self.order >= SIXTH else None
+        crk = np.zeros((ni, 3)) if self.order >= SIXTH else None
+        nnb = np.zeros(ni, np.int32); dsmin = np.full(ni, np.nan)
+        self._check(self._L.sapamd_get_grav_results(
+            self._h, int(nj), int(ni), _pd(acc), _pd(jrk), _pd(snp), _pd(crk), _pd(pot), _pi(nnb),
+            _pd(dsmin), 1 if ngb else 0), "get_grav_results")
+        return dict(acc=acc, jrk=jrk, snp=snp, crk=crk, pot=pot, nnb=nnb, dsmin=dsmin)
+
+    def calc(self, nj, ids, xi, vi, eps2, h2=None, ai=None, ngb=True):
+        """firsthalf + lasthalf in one call (any ni; chunked by n_pipes)."""
+        ni = len(ids)
+        out = None
+        for s in range(0, max(ni, 1), NPIPES):
+            e = min(ni, s + NPIPES)
+            self.start_grav_calc(nj, ids[s:e], xi[s:e], vi[s:e], eps2,
+                                 None if h2 is None else h2[s:e], None if ai is None else ai[s:e])
+            r = self.get_grav_results(nj, e - s, ngb)
+            if out is None:
+                out = {k: (None if v is None else np.empty((ni,) + v.shape[1:], v.dtype))
+                       for k, v in r.items()}
+            for k, v in r.items():
+                if v is not None:
+                    out[k][s:e] = v
+        return out
+
+    def read_ngb_list(self):
+        return self._check(self._L.sapamd_read_ngb_list(self._h), "read_ngb_list")
+
+    def get_ngb_list(self, ipipe, maxlen=NGB_CAP):
+        buf = np.zeros(max(maxlen, 1), np.int32)
+        n = ctypes.c_int(0)
+        ov = self._check(self._L.sapamd_get_ngb_list(self._h, int(ipipe), int(maxlen),
+                                                     ctypes.byref(n), _pi(buf)), "get_ngb_list")
+        return buf[:min(n.value, maxlen, NGB_CAP)].copy(), n.value, ov
+
+    def force_prediction(self, nj):
+        return self._check(self._L.sapamd_force_prediction(self._h, int(nj)), "force_prediction")
+
+    def retrieve_predicted_j_particle(self, addr):
+        m = ctypes.c_double(); idd = ctypes.c_double(); e2 = ctypes.c_double()
+        pos = np.zeros(3); vel = np.zeros(3); acc = np.zeros(3)
+        self._check(self._L.sapamd_retrieve_predicted_j_particle(
+            self._h, int(addr), ctypes.byref(m), ctypes.byref(idd), ctypes.byref(e2), _pd(pos),
+            _pd(vel), _pd(acc)), "retrieve_predicted_j_particle")
+        return dict(mass=m.value, id=int(idd.value), eps2=e2.value, pos=pos, vel=vel, acc=acc)
+
+    def retrieve_j_particle_state(self, addr):
+        m = ctypes.c_double(); idd = ctypes.c_double(); e2 = ctypes.c_double()
+        arrs = [np.zeros(3) for _ in range(7)]
+        self._check(self._L.sapamd_retrieve_j_particle_state(
+            self._h, int(addr), ctypes.byref(m), ctypes.byref(idd), ctypes.byref(e2),
+            *[_pd(a) for a in arrs]), "retrieve_j_particle_state")
+        keys = ["pos", "vel", "acc", "jrk", "ppos", "pvel", "pacc"]
+        d = dict(zip(keys, arrs))
+        d.update(mass=m.value, id=int(idd.value), eps2=e2.value)
+        return d
+
+    # --- launch shape / instrumentation -------------------------------------------------
+    def set_force_override(self, P=0, S=0):
+        self._L.sapamd_set_force_override(self._h, int(P), int(S))
+
+    def set_timing(self, on=True):
+        self._L.sapamd_set_timing(self._h, 1 if on else 0)
+
+    def last_force_ms(self):
+        return float(self._L.sapamd_last_force_ms(self._h))
+
+    def last_predict_ms(self):
+        return float(self._L.sapamd_last_predict_ms(self._h))
+
+    def last_config(self):
+        P = ctypes.c_int(); S = ctypes.c_int(); jc = ctypes.c_int()
+        self._L.sapamd_last_config(self._h, ctypes.byref(P), ctypes.byref(S), ctypes.byref(jc))
+        return P.value, S.value, jc.value
+
+    @property
+    def num_cus(self):
+        return self._L.sapamd_num_cus(self._h)
+
+    def synchronize(self):
+        return self._check(self._L.sapamd_synchronize(self._h), "synchronize")

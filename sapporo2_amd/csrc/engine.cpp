@@ -1,0 +1,682 @@
+// engine.cpp -- host engine (see engine.h).  Reference behaviour cited per method.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace sap {
+
+using namespace sapk;
+
+#define HIPCHK(call, what)                       \
+  do {                                           \
+    hipError_t e_ = (call);                      \
+    if (e_ != hipSuccess) return fail(what, e_); \
+  } while (0)
+
+namespace {
+constexpr int WAVES_PER_SIMD_TARGET = 8;
+constexpr int MIN_ITERS_PER_LANE = 16;
+constexpr int FOLD = 64;   // interactions per lane between FP64 folds (kernels.hip KT)
+constexpr int IBUF_FIELDS = 10;  // xi 3, vi 3, ai 3, h2 1
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+template <typename T>
+hipError_t hmalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipHostMalloc(reinterpret_cast<void**>(p), n * sizeof(T), hipHostMallocDefault);
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <typename T>
+void hfree(T*& p) {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+}
+}  // namespace
+
+Engine::Engine() = default;
+Engine::~Engine() {
+  if (open_) close();
+}
+
+int Engine::fail(const char* what, hipError_t e) {
+  err_ = what;
+  if (e != hipSuccess) {
+    err_ += ": ";
+    err_ += hipGetErrorString(e);
+  }
+  fprintf(stderr, "sapporo2_amd: %s\n", err_.c_str());
+  return -1;
+}
+
+// Reference sapporo::open (sapporohostclass.cpp:90-170): one device, kernels
+// are compiled in, initial j capacity 16384.
+int Engine::open(int device, int order, int precision) {
+  if (open_) close();
+  if (order != ORDER_H4 && order != ORDER_H6) return fail("unsupported integration order");
+  if (precision != PREC_DS && precision != PREC_DOUBLE) return fail("unsupported precision");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+  if (ndev <= 0) return fail("no HIP device");
+  if (device < 0 || device >= ndev) device = 0;
+  device_ = device;
+  order_ = order;
+  f64_ = precision == PREC_DOUBLE;
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
+  cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  HIPCHK(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_f0_), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_f1_), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_p0_), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_p1_), "hipEventCreate");
+  for (auto& s : stage_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
+
+  const size_t nI = NPIPES;
+  HIPCHK(hmalloc(&ibuf_h_, IBUF_FIELDS * nI), "pinned i buffer");
+  HIPCHK(hmalloc(&iid_h_, nI), "pinned i ids");
+  HIPCHK(dmalloc(&ibuf_d_, IBUF_FIELDS * nI), "i buffer");
+  HIPCHK(dmalloc(&iid_d_, nI), "i ids");
+  HIPCHK(dmalloc(&out_d_, nI * out_fields(order_)), "result buffer");
+  HIPCHK(hmalloc(&out_h_, nI * out_fields(order_)), "pinned results");
+  HIPCHK(dmalloc(&cnt_d_, nI), "neighbour counts");
+  HIPCHK(dmalloc(&list_d_, nI * NGB_CAP), "neighbour lists");
+  HIPCHK(dmalloc(&off_d_, nI + 1), "neighbour offsets");
+  HIPCHK(dmalloc(&packed_d_, nI * NGB_CAP), "packed lists");
+  HIPCHK(hmalloc(&off_h_, nI + 1), "pinned offsets");
+  HIPCHK(hmalloc(&packed_h_, nI * NGB_CAP), "pinned packed lists");
+  HIPCHK(hipMemsetAsync(cnt_d_, 0, nI * sizeof(int), stream_), "memset");
+
+  nj_max_ = NJ_INITIAL;
+  max_addr_ = -1;
+  slot_of_.assign(nj_max_, -1);
+  nstaged_ = 0;
+  cur_ = 0;
+  predict_ = false;
+  t_i_ = 0.0;
+  open_ = true;
+  if (ensure_store(NJ_INITIAL) != 0) return -1;
+  if (grow_staging(1024) != 0) return -1;
+  HIPCHK(hipStreamSynchronize(stream_), "open sync");
+  return 0;
+}
+
+// Reference sapporo::close (sapporohostclass.cpp:391-402).
+int Engine::close() {
+  if (!open_) return 0;
+  (void)hipSetDevice(device_);
+  (void)hipStreamSynchronize(stream_);
+  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc);
+  st_.cap = 0;
+  for (auto& s : stage_) {
+    hfree(s.d); hfree(s.ai); s.cap = 0; s.busy = false;
+    if (s.ev) (void)hipEventDestroy(s.ev);
+    s.ev = nullptr;
+  }
+  dfree(upd_d_); dfree(upd_ai_d_); upd_cap_d_ = 0;
+  hfree(ibuf_h_); hfree(iid_h_); dfree(ibuf_d_); dfree(iid_d_);
+  dfree(part_d_); dfree(pds2_d_); dfree(paddr_d_); part_cap_ = 0;
+  dfree(out_d_); hfree(out_h_);
+  dfree(cnt_d_); dfree(list_d_); dfree(off_d_); dfree(packed_d_);
+  hfree(off_h_); hfree(packed_h_);
+  dfree(gather_d_); gather_cap_ = 0;
+  for (hipEvent_t* e : {&ev_done_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
+    if (*e) (void)hipEventDestroy(*e);
+    *e = nullptr;
+  }
+  if (stream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
+  open_ = inflight_ = have_results_ = ngb_fetched_ = jhost_valid_ = false;
+  slot_of_.clear();
+  nstaged_ = 0;
+  return 0;
+}
+
+int Engine::synchronize() {
+  if (!open_) return 0;
+  HIPCHK(hipStreamSynchronize(stream_), "synchronize");
+  inflight_ = false;
+  return 0;
+}
+
+// Device j-store growth.  New slots are zeroed (massless, at the origin).
+int Engine::ensure_store(int need) {
+  if (need <= st_.cap) return 0;
+  const int oldcap = st_.cap;
+  long nc = std::max<long>({(long)need, (long)oldcap + oldcap / 2, (long)NJ_INITIAL});
+  nc = (nc + 255) / 256 * 256;
+  const int newcap = (int)nc;
+  const int nfr = raw_fields(order_);
+  const size_t rb = jrec_bytes(order_, f64_);
+  StoreView ns{};
+  ns.cap = newcap;
+  HIPCHK(dmalloc(&ns.raw, (size_t)nfr * newcap), "j-store raw");
+  HIPCHK(dmalloc(&ns.id, (size_t)newcap), "j-store ids");
+  HIPCHK(hipMalloc(&ns.rec, rb * newcap), "j-store records");
+  if (!f64_) {
+    HIPCHK(dmalloc(&ns.pvel, (size_t)3 * newcap), "j-store pvel");
+    if (order_ >= ORDER_H6) HIPCHK(dmalloc(&ns.pacc, (size_t)3 * newcap), "j-store pacc");
+  }
+  HIPCHK(hipMemsetAsync(ns.raw, 0, sizeof(double) * nfr * (size_t)newcap, stream_), "memset");
+  HIPCHK(hipMemsetAsync(ns.id, 0, sizeof(int) * (size_t)newcap, stream_), "memset");
+  HIPCHK(hipMemsetAsync(ns.rec, 0, rb * newcap, stream_), "memset");
+  if (ns.pvel) HIPCHK(hipMemsetAsync(ns.pvel, 0, sizeof(double) * 3 * (size_t)newcap, stream_), "memset");
+  if (ns.pacc) HIPCHK(hipMemsetAsync(ns.pacc, 0, sizeof(double) * 3 * (size_t)newcap, stream_), "memset");
+  if (oldcap > 0) {
+    for (int f = 0; f < nfr; ++f)
+      HIPCHK(hipMemcpyAsync(ns.raw + (size_t)f * newcap, st_.raw + (size_t)f * oldcap,
+                            sizeof(double) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
+    HIPCHK(hipMemcpyAsync(ns.id, st_.id, sizeof(int) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
+    HIPCHK(hipMemcpyAsync(ns.rec, st_.rec, rb * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
+    for (int d = 0; d < 3; ++d) {
+      if (ns.pvel)
+        HIPCHK(hipMemcpyAsync(ns.pvel + (size_t)d * newcap, st_.pvel + (size_t)d * oldcap,
+                              sizeof(double) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
+      if (ns.pacc)
+        HIPCHK(hipMemcpyAsync(ns.pacc + (size_t)d * newcap, st_.pacc + (size_t)d * oldcap,
+                              sizeof(double) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
+    }
+    HIPCHK(hipStreamSynchronize(stream_), "store grow sync");
+  }
+  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc);
+  st_ = ns;
+  return 0;
+}
+
+int Engine::grow_staging(int need) {
+  const int nf = raw_fields(order_);
+  for (auto& s : stage_) {
+    if (s.cap >= need) continue;
+    if (s.busy) {
+      HIPCHK(hipEventSynchronize(s.ev), "staging wait");
+      s.busy = false;
+    }
+    const int nc = std::max(need, s.cap * 2);
+    double* d = nullptr;
+    int* ai = nullptr;
+    HIPCHK(hmalloc(&d, (size_t)nc * nf), "pinned staging");
+    HIPCHK(hmalloc(&ai, (size_t)nc * 2), "pinned staging ids");
+    if (s.cap > 0 && &s == &stage_[cur_] && nstaged_ > 0) {
+      memcpy(d, s.d, sizeof(double) * (size_t)nstaged_ * nf);
+      memcpy(ai, s.ai, sizeof(int) * (size_t)nstaged_ * 2);
+    }
+    hfree(s.d);
+    hfree(s.ai);
+    s.d = d;
+    s.ai = ai;
+    s.cap = nc;
+  }
+  return 0;
+}
+
+// Reference set_time / set_no_time (sapporohostclass.cpp:405-427).
+int Engine::set_time(double t) {
+  t_i_ = t;
+  predict_ = true;
+  jhost_valid_ = false;
+  return 0;
+}
+int Engine::set_no_time() {
+  predict_ = false;
+  jhost_valid_ = false;
+  return 0;
+}
+
+// Reference set_j_particle (sapporohostclass.cpp:434-558): stage the update;
+// re-setting an address before the next flush overwrites its slot; the
+// logical capacity grows by 10% and addresses beyond that are rejected.
+int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass, const double a[3],
+                           const double jrk[3], const double v[3], const double x[3],
+                           const double snp[3], const double crk[3], double eps) {
+  if (!open_) return fail("set_j_particle: library not open");
+  if (addr < 0) return fail("set_j_particle: negative address");
+  if (addr >= nj_max_) {
+    fprintf(stderr, "sapporo2_amd: increasing nj_max from %d for address %d\n", nj_max_, addr);
+    nj_max_ = (int)(nj_max_ * 1.1) + 1;
+    if (addr >= nj_max_) {
+      fprintf(stderr, "sapporo2_amd: address %d beyond the grown capacity %d; send particles in order\n",
+              addr, nj_max_);
+      return fail("set_j_particle: address out of order");
+    }
+  }
+  if ((int)slot_of_.size() < nj_max_) slot_of_.resize(nj_max_, -1);
+  Stage* sg = &stage_[cur_];
+  if (sg->busy) {
+    HIPCHK(hipEventSynchronize(sg->ev), "staging wait");
+    sg->busy = false;
+  }
+  int k = slot_of_[addr];
+  if (k < 0) {
+    if (nstaged_ >= sg->cap) {
+      if (grow_staging(nstaged_ + 1) != 0) return -1;
+      sg = &stage_[cur_];
+    }
+    k = nstaged_++;
+    slot_of_[addr] = k;
+  }
+  const int nf = raw_fields(order_);
+  double* u = sg->d + (size_t)k * nf;
+  u[RF_T] = tj; u[RF_DT] = dtj; u[RF_M] = mass; u[RF_EPS] = eps;
+  for (int d = 0; d < 3; ++d) {
+    u[RF_X + d] = x ? x[d] : 0.0;
+    u[RF_VX + d] = v ? v[d] : 0.0;
+    u[RF_AX + d] = a ? a[d] : 0.0;
+    u[RF_JX + d] = jrk ? jrk[d] : 0.0;
+    if (order_ >= ORDER_H6) {
+      u[RF_SX + d] = snp ? snp[d] : 0.0;
+      u[RF_CX + d] = crk ? crk[d] : 0.0;
+    }
+  }
+  sg->ai[2 * k] = addr;
+  sg->ai[2 * k + 1] = id;
+  if (addr > max_addr_) max_addr_ = addr;
+  jhost_valid_ = false;
+  return 0;
+}
+
+// Reference send_j_particles_to_device + copyJInDev (sapporohostclass.cpp:889-923,
+// 1204-1248), fused with the neighbour-counter reset of dev_reset_buffers.
+int Engine::flush_updates(int ni_reset) {
+  const int n = nstaged_;
+  const int nf = raw_fields(order_);
+  if (n > 0) {
+    if (ensure_store(max_addr_ + 1) != 0) return -1;
+    if (upd_cap_d_ < n) {
+      dfree(upd_d_);
+      dfree(upd_ai_d_);
+      const int nc = std::max(n, upd_cap_d_ * 2);
+      HIPCHK(dmalloc(&upd_d_, (size_t)nc * nf), "device staging");
+      HIPCHK(dmalloc(&upd_ai_d_, (size_t)nc * 2), "device staging ids");
+      upd_cap_d_ = nc;
+    }
+    Stage& sg = stage_[cur_];
+    HIPCHK(hipMemcpyAsync(upd_d_, sg.d, sizeof(double) * (size_t)n * nf, hipMemcpyHostToDevice, stream_),
+           "update upload");
+    HIPCHK(hipMemcpyAsync(upd_ai_d_, sg.ai, sizeof(int) * (size_t)n * 2, hipMemcpyHostToDevice, stream_),
+           "update upload");
+    HIPCHK(hipEventRecord(sg.ev, stream_), "event");
+    sg.busy = true;
+    for (int k = 0; k < n; ++k) slot_of_[sg.ai[2 * k]] = -1;
+    nstaged_ = 0;
+    cur_ ^= 1;
+  }
+  if (n > 0 || ni_reset > 0) {
+    ScatterArgs a{};
+    a.st = st_;
+    a.upd = upd_d_;
+    a.upd_ai = upd_ai_d_;
+    a.nupd = n;
+    a.nf = nf;
+    a.ngb_cnt = cnt_d_;
+    a.ni_reset = ni_reset;
+    if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
+  }
+  return 0;
+}
+
+int Engine::predict_range(int j0, int j1) {
+  if (j1 <= j0) return 0;
+  if (ensure_store(j1) != 0) return -1;
+  PredictArgs p{};
+  p.st = st_;
+  p.j0 = j0;
+  p.j1 = j1;
+  p.ti = t_i_;
+  if (timing_) HIPCHK(hipEventRecord(ev_p0_, stream_), "event");
+  if (launch_predict(stream_, order_, f64_, p) != 0) return fail("predict launch", hipGetLastError());
+  if (timing_) HIPCHK(hipEventRecord(ev_p1_, stream_), "event");
+  timed_predict_ = timing_;
+  return 0;
+}
+
+int Engine::ensure_partials(size_t entries) {
+  if (entries <= part_cap_) return 0;
+  dfree(part_d_);
+  dfree(pds2_d_);
+  dfree(paddr_d_);
+  const size_t nc = std::max(entries, part_cap_ + part_cap_ / 2);
+  HIPCHK(dmalloc(&part_d_, nc * part_fields(order_)), "partials");
+  HIPCHK(dmalloc(&pds2_d_, nc), "partials");
+  HIPCHK(dmalloc(&paddr_d_, nc), "partials");
+  part_cap_ = nc;
+  return 0;
+}
+
+// Launch shape: P i-particles per wave (64 unless fewer are active), S j-splits
+// so that about WAVES_PER_SIMD_TARGET waves per SIMD exist, each split at least
+// MIN_ITERS_PER_LANE iterations per lane.
+ForceConfig Engine::choose_config(int ni, int nj) const {
+  int P = 64;
+  if (ni < 64) {
+    P = 1;
+    while (P < ni) P <<= 1;
+  }
+  if (ovr_P_ > 0 && force_p_supported(ovr_P_)) P = ovr_P_;
+  const int Q = 64 / P;
+  const long G = (ni + P - 1) / P;
+  const long target = (long)cus_ * 4 * WAVES_PER_SIMD_TARGET;
+  long S = (target + G - 1) / G;
+  const long maxS = std::max<long>(1, (nj + (long)Q * MIN_ITERS_PER_LANE - 1) / ((long)Q * MIN_ITERS_PER_LANE));
+  S = std::min(S, maxS);
+  if (ovr_S_ > 0) S = ovr_S_;
+  if (S < 1) S = 1;
+  long jchunk = (nj + S - 1) / S;
+  const long tile = (long)Q * FOLD;
+  if (jchunk >= tile) jchunk = (jchunk + tile - 1) / tile * tile;
+  else jchunk = (jchunk + Q - 1) / Q * Q;
+  if (jchunk < 1) jchunk = 1;
+  S = (nj + jchunk - 1) / jchunk;
+  if (S < 1) S = 1;
+  return ForceConfig{P, (int)S, (int)jchunk};
+}
+
+// Reference startGravCalc (sapporohostclass.cpp:586-703).
+int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3], const double (*vi)[3],
+                            const double (*ai)[3], double eps2, const double* h2) {
+  if (!open_) return fail("start_grav_calc: library not open");
+  if (ni == 0 || nj == 0) return 0;  // reference 597
+  if (ni < 0 || nj < 0) return fail("start_grav_calc: negative count");
+  if (ni > NPIPES) return fail("start_grav_calc: ni exceeds n_pipes");  // reference assert, 604
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (inflight_ && wait_results() != 0) return -1;
+  if (ensure_store(nj) != 0) return -1;
+
+  // pack i-particles: [xi | vi | ai | h2] doubles and ids (reference 613-649)
+  double* bx = ibuf_h_;
+  double* bv = bx + 3 * (size_t)ni;
+  double* ba = bv + 3 * (size_t)ni;
+  double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
+  bool ngb_on = false;
+  for (int i = 0; i < ni; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      bx[3 * i + d] = xi[i][d];
+      bv[3 * i + d] = vi ? vi[i][d] : 0.0;
+      if (order_ >= ORDER_H6) ba[3 * i + d] = ai ? ai[i][d] : 0.0;
+    }
+    const double hh = h2 ? h2[i] : 0.0;
+    bh[i] = hh;
+    ngb_on |= hh > 0.0;
+    iid_h_[i] = id ? id[i] : -1;
+  }
+  const size_t nd = (size_t)(bh + ni - bx);
+  HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
+  HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
+
+  if (flush_updates(ngb_on ? ni : 0) != 0) return -1;
+  if (predict_) {
+    if (predict_range(0, nj) != 0) return -1;
+  } else {
+    timed_predict_ = false;
+  }
+  predict_ = false;  // reference 698
+
+  const ForceConfig cfg = choose_config(ni, nj);
+  if (ensure_partials((size_t)cfg.S * ni) != 0) return -1;
+  ForceArgs fa{};
+  fa.rec = st_.rec;
+  fa.nj = nj;
+  fa.ipos = ibuf_d_;
+  fa.ivel = ibuf_d_ + 3 * (size_t)ni;
+  fa.iacc = ibuf_d_ + 6 * (size_t)ni;
+  fa.ih2 = ibuf_d_ + (bh - bx);
+  fa.iid = iid_d_;
+  fa.ni = ni;
+  fa.eps2 = eps2;
+  fa.part = part_d_;
+  fa.part_ds2 = pds2_d_;
+  fa.part_addr = paddr_d_;
+  fa.ngb_cnt = cnt_d_;
+  fa.ngb_list = list_d_;
+  fa.ngb_on = ngb_on ? 1 : 0;
+  if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
+  const int rc = launch_force(stream_, order_, f64_, fa, cfg);
+  if (rc != 0) return fail("force launch", hipGetLastError());
+  if (timing_) HIPCHK(hipEventRecord(ev_f1_, stream_), "event");
+
+  FinalizeArgs fz{};
+  fz.part = part_d_;
+  fz.part_ds2 = pds2_d_;
+  fz.part_addr = paddr_d_;
+  fz.S = cfg.S;
+  fz.ni = ni;
+  fz.rec = st_.rec;
+  fz.ngb_cnt = cnt_d_;
+  fz.ngb_list = list_d_;
+  fz.out = out_d_;
+  fz.ngb_on = fa.ngb_on;
+  if (launch_finalize(stream_, order_, f64_, fz) != 0) return fail("finalize launch", hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
+                        hipMemcpyDeviceToHost, stream_), "result download");
+  HIPCHK(hipEventRecord(ev_done_, stream_), "event");
+  inflight_ = true;
+  have_results_ = true;
+  ngb_fetched_ = false;
+  ngb_on_last_ = ngb_on;
+  ni_last_ = ni;
+  nj_last_ = nj;
+  last_cfg_ = cfg;
+  jhost_valid_ = false;
+  return 0;
+}
+
+int Engine::wait_results() {
+  if (!inflight_) return 0;
+  HIPCHK(hipEventSynchronize(ev_done_), "result wait");
+  HIPCHK(hipGetLastError(), "device error");
+  inflight_ = false;
+  return 0;
+}
+
+float Engine::last_force_ms() {
+  if (!timing_ || !have_results_) return 0.f;
+  (void)hipEventSynchronize(ev_f1_);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ev_f0_, ev_f1_) != hipSuccess) return 0.f;
+  return ms;
+}
+
+float Engine::last_predict_ms() {
+  if (!timing_ || !timed_predict_) return 0.f;
+  (void)hipEventSynchronize(ev_p1_);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ev_p0_, ev_p1_) != hipSuccess) return 0.f;
+  return ms;
+}
+
+// Reference getGravResults (sapporohostclass.cpp:705-802): outputs are
+// written for [0, ni); the nearest neighbour is reported only when its ds2
+// (float) is below 1e10, otherwise nnb = 0 and dsmin is left untouched.
+int Engine::get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3], double (*snp)[3],
+                             double (*crk)[3], double* pot, int* nnb, double* dsmin, bool ngb) {
+  if (!open_) return fail("get_grav_results: library not open");
+  if (ni == 0 || nj == 0) return 0;  // reference 718
+  if (!have_results_ || ni > ni_last_) return fail("get_grav_results: no matching firsthalf");
+  if (wait_results() != 0) return -1;
+  const int NO = out_fields(order_);
+  const int NF = part_fields(order_);
+  for (int i = 0; i < ni; ++i) {
+    const double* o = out_h_ + (size_t)i * NO;
+    if (acc) { acc[i][0] = o[0]; acc[i][1] = o[1]; acc[i][2] = o[2]; }
+    if (pot) pot[i] = o[3];
+    if (jrk) { jrk[i][0] = o[4]; jrk[i][1] = o[5]; jrk[i][2] = o[6]; }
+    if (order_ >= ORDER_H6) {
+      if (snp) { snp[i][0] = o[7]; snp[i][1] = o[8]; snp[i][2] = o[9]; }
+      if (crk) { crk[i][0] = crk[i][1] = crk[i][2] = 0.0; }
+    }
+    if (ngb) {
+      if (nnb) nnb[i] = 0;
+      const double ds = o[NF];
+      if (ds < 1.0e10) {
+        if (nnb) nnb[i] = (int)o[NF + 1];
+        if (dsmin) dsmin[i] = ds;
+      }
+    }
+  }
+  return 0;
+}
+
+// Reference read_ngb_list (sapporohostclass.cpp:804-835): fetch the lists of
+// the last call; returns 1 if any i-particle has >= 256 neighbours.
+int Engine::read_ngb_list() {
+  if (!open_ || !have_results_) return 0;
+  if (wait_results() != 0) return -1;
+  const int ni = ni_last_;
+  const int NO = out_fields(order_);
+  const int NF = part_fields(order_);
+  if (ngb_on_last_) {
+    PackArgs pa{};
+    pa.ngb_cnt = cnt_d_;
+    pa.ngb_list = list_d_;
+    pa.ni = ni;
+    pa.offsets = off_d_;
+    pa.packed = packed_d_;
+    if (launch_pack(stream_, pa) != 0) return fail("pack launch", hipGetLastError());
+    HIPCHK(hipMemcpyAsync(off_h_, off_d_, sizeof(int) * (size_t)(ni + 1), hipMemcpyDeviceToHost, stream_),
+           "offset download");
+    HIPCHK(hipStreamSynchronize(stream_), "offset sync");
+    const int total = off_h_[ni];
+    if (total > 0)
+      HIPCHK(hipMemcpyAsync(packed_h_, packed_d_, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, stream_),
+             "list download");
+    HIPCHK(hipStreamSynchronize(stream_), "list sync");
+  } else {
+    for (int i = 0; i <= ni; ++i) off_h_[i] = 0;
+  }
+  ngb_fetched_ = true;
+  int overflow = 0;
+  for (int i = 0; i < ni; ++i)
+    if (out_h_[(size_t)i * NO + NF + 2] >= NGB_CAP) overflow = 1;
+  return overflow;
+}
+
+// Reference get_ngb_list (sapporohostclass.cpp:837-880): copies up to
+// maxlength ids (sorted ascending), n = the neighbour count, returns 1 when
+// the count reaches maxlength.
+int Engine::get_ngb_list(int ipipe, int maxlen, int* n, int* list) {
+  if (n) *n = 0;
+  if (!open_ || !have_results_) return 0;
+  if (!ngb_fetched_) {
+    const int r = read_ngb_list();
+    if (r < 0) return r;
+  }
+  if (ipipe < 0 || ipipe >= ni_last_) return fail("get_ngb_list: ipipe out of range");
+  const int NO = out_fields(order_);
+  const int NF = part_fields(order_);
+  const int len = (int)out_h_[(size_t)ipipe * NO + NF + 2];
+  const int c = std::min(len, NGB_CAP);
+  const int ncopy = std::max(0, std::min(c, maxlen));
+  if (list && ncopy > 0) memcpy(list, packed_h_ + off_h_[ipipe], sizeof(int) * (size_t)ncopy);
+  if (n) *n = len;
+  const bool overflow = len >= maxlen;
+  if (overflow)
+    fprintf(stderr, "sapporo2_amd: get_ngb_list overflow for ipipe= %d, ngb= %d\n", ipipe, len);
+  return overflow ? 1 : 0;
+}
+
+// Reference forcePrediction (sapporohostclass.cpp:1164-1195).
+int Engine::force_prediction(int nj) {
+  if (!open_) return fail("force_prediction: library not open");
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (inflight_ && wait_results() != 0) return -1;
+  if (flush_updates(0) != 0) return -1;
+  if (predict_ && nj > 0) {
+    if (predict_range(0, nj) != 0) return -1;
+  }
+  predict_ = false;
+  jhost_valid_ = false;
+  return 0;
+}
+
+// Read-back of the j-store (predicted and raw state) for [0, max address].
+int Engine::fetch_j_host(int need) {
+  if (jhost_valid_ && need <= jhost_n_) return 0;
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (inflight_ && wait_results() != 0) return -1;
+  if (flush_updates(0) != 0) return -1;
+  const int n = std::max(need, max_addr_ + 1);
+  if (n <= 0) return 0;
+  if (ensure_store(n) != 0) return -1;
+  if (gather_cap_ < (size_t)n) {
+    dfree(gather_d_);
+    HIPCHK(dmalloc(&gather_d_, (size_t)n * 10), "gather buffer");
+    gather_cap_ = n;
+  }
+  GatherArgs g{};
+  g.st = st_;
+  g.n = n;
+  g.out = gather_d_;
+  if (launch_gather_pred(stream_, order_, f64_, g) != 0) return fail("gather launch", hipGetLastError());
+  const int nfr = raw_fields(order_);
+  jpred_h_.resize((size_t)n * 10);
+  jraw_h_.resize((size_t)n * nfr);
+  jid_h_.resize(n);
+  HIPCHK(hipMemcpyAsync(jpred_h_.data(), gather_d_, sizeof(double) * (size_t)n * 10, hipMemcpyDeviceToHost,
+                        stream_), "gather download");
+  for (int f = 0; f < nfr; ++f)
+    HIPCHK(hipMemcpyAsync(jraw_h_.data() + (size_t)f * n, st_.raw + (size_t)f * st_.cap, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, stream_), "raw download");
+  HIPCHK(hipMemcpyAsync(jid_h_.data(), st_.id, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, stream_),
+         "id download");
+  HIPCHK(hipStreamSynchronize(stream_), "read-back sync");
+  jhost_valid_ = true;
+  jhost_n_ = n;
+  return 0;
+}
+
+// Reference retrieve_predicted_j_particle (sapporohostclass.cpp:969-1041).
+int Engine::retrieve_predicted_j_particle(int addr, double* mass, double* id, double* eps2, double pos[3],
+                                          double vel[3], double acc[3]) {
+  if (!open_) return fail("retrieve_predicted_j_particle: library not open");
+  if (addr < 0) return fail("retrieve_predicted_j_particle: negative address");
+  if (fetch_j_host(addr + 1) != 0) return -1;
+  if (addr >= jhost_n_) return fail("retrieve_predicted_j_particle: address never set");
+  const double* p = jpred_h_.data() + (size_t)addr * 10;
+  if (pos) { pos[0] = p[0]; pos[1] = p[1]; pos[2] = p[2]; }
+  if (mass) *mass = p[3];
+  if (vel) { vel[0] = p[4]; vel[1] = p[5]; vel[2] = p[6]; }
+  if (eps2) *eps2 = jraw_h_[(size_t)RF_EPS * jhost_n_ + addr];
+  if (acc && order_ >= ORDER_H6) { acc[0] = p[7]; acc[1] = p[8]; acc[2] = p[9]; }
+  if (id) *id = (double)jid_h_[addr];
+  return 0;
+}
+
+// Reference retrieve_j_particle_state (sapporohostclass.cpp:1045-1147).
+int Engine::retrieve_j_particle_state(int addr, double* mass, double* id, double* eps2, double pos[3],
+                                      double vel[3], double acc[3], double jrk[3], double ppos[3],
+                                      double pvel[3], double pacc[3]) {
+  if (!open_) return fail("retrieve_j_particle_state: library not open");
+  if (addr < 0) return fail("retrieve_j_particle_state: negative address");
+  if (fetch_j_host(addr + 1) != 0) return -1;
+  if (addr >= jhost_n_) return fail("retrieve_j_particle_state: address never set");
+  const size_t n = jhost_n_;
+  auto raw = [&](int f) { return jraw_h_[(size_t)f * n + addr]; };
+  const double* p = jpred_h_.data() + (size_t)addr * 10;
+  if (mass) *mass = raw(RF_M);
+  if (pos) for (int d = 0; d < 3; ++d) pos[d] = raw(RF_X + d);
+  if (ppos) for (int d = 0; d < 3; ++d) ppos[d] = p[d];
+  if (pvel) for (int d = 0; d < 3; ++d) pvel[d] = p[4 + d];
+  if (eps2) *eps2 = raw(RF_EPS);
+  if (vel) for (int d = 0; d < 3; ++d) vel[d] = raw(RF_VX + d);
+  if (acc) for (int d = 0; d < 3; ++d) acc[d] = raw(RF_AX + d);
+  if (jrk) for (int d = 0; d < 3; ++d) jrk[d] = raw(RF_JX + d);
+  if (id) *id = (double)jid_h_[addr];
+  if (pacc && order_ >= ORDER_H6) for (int d = 0; d < 3; ++d) pacc[d] = p[7 + d];
+  return 0;
+}
+
+}  // namespace sap

@@ -1,0 +1,137 @@
+// engine.h -- host engine of the direct-summation library.
+//
+// One Engine drives one GPU.  It mirrors the reference's host class
+// (lib/include/sapporohostclass.h:121-176, lib/src/sapporohostclass.cpp) with
+// the same call sequence and result semantics, re-designed for one MI355X:
+//   * j-particles live in an FP64 field-major store in HBM; updates are staged
+//     in pinned host memory and sent with one asynchronous copy per flush,
+//     then scattered by a kernel (no per-particle map lookups on the device);
+//   * each force call is scatter -> [predict] -> force -> finalize on one HIP
+//     stream, followed by one asynchronous result copy; lasthalf waits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace sap {
+
+constexpr int NPIPES = 16384;        // reference defines.h:13-15
+constexpr int NJ_INITIAL = 16384;    // reference sapporohostclass.cpp:161-162
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // order: sapk::ORDER_H4 / ORDER_H6; precision: PREC_DS (mixed) or PREC_DOUBLE.
+  int open(int device, int order, int precision);
+  int close();
+  bool is_open() const { return open_; }
+  int order() const { return order_; }
+  int precision() const { return f64_ ? sapk::PREC_DOUBLE : sapk::PREC_DS; }
+  int n_pipes() const { return NPIPES; }
+
+  int set_time(double t);
+  int set_no_time();
+  int set_j_particle(int addr, int id, double tj, double dtj, double mass, const double a[3],
+                     const double jrk[3], const double v[3], const double x[3], const double snp[3],
+                     const double crk[3], double eps);
+  int start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3], const double (*vi)[3],
+                      const double (*ai)[3], double eps2, const double* h2);
+  int get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3], double (*snp)[3],
+                       double (*crk)[3], double* pot, int* nnb, double* dsmin, bool ngb);
+  int read_ngb_list();
+  int get_ngb_list(int ipipe, int maxlen, int* n, int* list);
+  int force_prediction(int nj);
+  int retrieve_predicted_j_particle(int addr, double* mass, double* id, double* eps2, double pos[3],
+                                    double vel[3], double acc[3]);
+  int retrieve_j_particle_state(int addr, double* mass, double* id, double* eps2, double pos[3],
+                                double vel[3], double acc[3], double jrk[3], double ppos[3],
+                                double pvel[3], double pacc[3]);
+
+  // Launch-shape control and instrumentation (not part of the reference API).
+  void set_force_override(int P, int S) { ovr_P_ = P; ovr_S_ = S; }
+  sapk::ForceConfig choose_config(int ni, int nj) const;
+  void set_timing(bool on) { timing_ = on; }
+  float last_force_ms();          // force kernel time of the last call (timing on)
+  float last_predict_ms();        // predictor time of the last call (timing on, 0 if none)
+  sapk::ForceConfig last_config() const { return last_cfg_; }
+  int device() const { return device_; }
+  int num_cus() const { return cus_; }
+  hipStream_t stream() const { return stream_; }
+  const char* last_error() const { return err_.c_str(); }
+  int synchronize();
+
+ private:
+  int fail(const char* what, hipError_t e = hipSuccess);
+  int ensure_store(int need);
+  int ensure_partials(size_t entries);
+  int flush_updates(int ni_reset);
+  int predict_range(int j0, int j1);
+  int wait_results();
+  int fetch_j_host(int nj);
+  int grow_staging(int need);
+
+  bool open_ = false;
+  int device_ = 0, order_ = sapk::ORDER_H4, cus_ = 256;
+  bool f64_ = false;
+  hipStream_t stream_ = nullptr;
+  std::string err_;
+
+  // j-store
+  sapk::StoreView st_{};
+  int nj_max_ = NJ_INITIAL;   // logical capacity (reference growth rule)
+  int max_addr_ = -1;         // largest address ever stored
+
+  // staged updates (pinned), double-buffered
+  struct Stage {
+    double* d = nullptr;      // cap x nf doubles
+    int* ai = nullptr;        // cap x 2 ints
+    int cap = 0;
+    hipEvent_t ev = nullptr;  // completion of the copy that last read it
+    bool busy = false;
+  };
+  Stage stage_[2];
+  int cur_ = 0, nstaged_ = 0;
+  std::vector<int> slot_of_;  // address -> staged slot or -1
+  double* upd_d_ = nullptr;
+  int* upd_ai_d_ = nullptr;
+  int upd_cap_d_ = 0;
+
+  // prediction state
+  double t_i_ = 0.0;
+  bool predict_ = false;
+
+  // i-particles and results
+  double* ibuf_h_ = nullptr; int* iid_h_ = nullptr;
+  double* ibuf_d_ = nullptr; int* iid_d_ = nullptr;
+  double* part_d_ = nullptr; double* pds2_d_ = nullptr; int* paddr_d_ = nullptr;
+  size_t part_cap_ = 0;
+  double* out_d_ = nullptr; double* out_h_ = nullptr;
+  int* cnt_d_ = nullptr; int* list_d_ = nullptr;
+  int* off_d_ = nullptr; int* packed_d_ = nullptr;
+  int* off_h_ = nullptr; int* packed_h_ = nullptr;
+  hipEvent_t ev_done_ = nullptr, ev_f0_ = nullptr, ev_f1_ = nullptr, ev_p0_ = nullptr, ev_p1_ = nullptr;
+  bool inflight_ = false, have_results_ = false, ngb_fetched_ = false, ngb_on_last_ = false;
+  bool timed_predict_ = false;
+  int ni_last_ = 0, nj_last_ = 0;
+  sapk::ForceConfig last_cfg_{64, 1, 1};
+  bool timing_ = false;
+  int ovr_P_ = 0, ovr_S_ = 0;
+
+  // read-back cache of the j-store (predicted and raw state)
+  bool jhost_valid_ = false;
+  int jhost_n_ = 0;
+  std::vector<double> jpred_h_, jraw_h_;
+  std::vector<int> jid_h_;
+  double* gather_d_ = nullptr; size_t gather_cap_ = 0;
+};
+
+}  // namespace sap

@@ -1,0 +1,132 @@
+// kernels.h -- interface between the host engine and the HIP kernels.
+//
+// The engine (engine.cpp) is the only caller; it owns every device buffer and
+// passes plain pointers.  All kernels are written for gfx950 (wave64, 256 CUs).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace sapk {
+
+// Integration order / working precision, numbered as the reference's enums
+// (lib/include/defines.h:30-31).
+enum Order { ORDER_G5 = 0, ORDER_H4 = 1, ORDER_H6 = 2 };
+enum Precision { PREC_FLOAT = 0, PREC_DS = 1, PREC_DOUBLE = 2 };
+
+// Per-i neighbour slots: the reference's NGB_PB (defines.h:9).
+constexpr int NGB_CAP = 256;
+
+// Raw j-store: one double array per field ("field-major SoA"), each of the
+// store's capacity.  Fields follow the reference's set_j_particle payload
+// (sapporohostclass.cpp:500-516): time, dt, mass, eps, pos, vel, acc, jerk,
+// snap, crackle.
+enum RawField {
+  RF_T = 0, RF_DT, RF_M, RF_EPS,
+  RF_X, RF_Y, RF_Z, RF_VX, RF_VY, RF_VZ,
+  RF_AX, RF_AY, RF_AZ, RF_JX, RF_JY, RF_JZ,
+  RF_SX, RF_SY, RF_SZ, RF_CX, RF_CY, RF_CZ,
+  RF_COUNT
+};
+// Number of raw fields an update record carries for an order.
+constexpr int raw_fields(int order) { return order >= ORDER_H6 ? RF_COUNT : RF_SX; }
+
+// Predicted j-records read by the force kernels (one array of structs, read
+// wave-uniformly through the scalar cache or per lane).  Positions stay in
+// FP64 in every variant: relative positions are formed by FP64 subtraction.
+struct JRecM4 { double x, y, z; float m, vx, vy, vz; int id, pad; };                  // 48 B
+struct JRecM6 { double x, y, z; float m, vx, vy, vz, ax, ay, az; int id, pad[2]; };   // 64 B
+struct JRecD4 { double x, y, z, m, vx, vy, vz; int id, pad; };                        // 64 B
+struct JRecD6 { double x, y, z, m, vx, vy, vz, ax, ay, az; int id, pad[3]; };         // 96 B
+
+inline size_t jrec_bytes(int order, bool f64) {
+  if (order >= ORDER_H6) return f64 ? sizeof(JRecD6) : sizeof(JRecM6);
+  return f64 ? sizeof(JRecD4) : sizeof(JRecM4);
+}
+
+// Number of doubles per i-particle result record written by the finalize
+// kernel: acc[3], pot, jerk[3], [snap[3]], dsmin, nnb (as double), count.
+constexpr int out_fields(int order) { return order >= ORDER_H6 ? 13 : 10; }
+// Number of FP64 partial-force fields: acc[3], pot, jerk[3], [snap[3]].
+constexpr int part_fields(int order) { return order >= ORDER_H6 ? 10 : 7; }
+
+// ---------------------------------------------------------------------------
+struct StoreView {
+  double* raw;       // RF_COUNT (or raw_fields) x cap doubles
+  int* id;           // cap ints
+  void* rec;         // cap predicted records
+  double* pvel;      // mixed precision only: 3 x cap FP64 predicted velocity
+  double* pacc;      // mixed precision SIXTH only: 3 x cap FP64 predicted acc
+  int cap;
+};
+
+// Scatter staged updates into the store and reset the per-i neighbour
+// counters of the coming force call (fused: one launch).
+struct ScatterArgs {
+  StoreView st;
+  const double* upd;     // nupd x nf doubles (fields 0..nf-1 of RawField)
+  const int* upd_ai;     // nupd x 2 ints: address, id
+  int nupd, nf;
+  int* ngb_cnt;          // zeroed for [0, ni_reset)
+  int ni_reset;
+};
+
+struct PredictArgs {
+  StoreView st;
+  int j0, j1;            // predict addresses [j0, j1)
+  double ti;
+};
+
+struct ForceArgs {
+  const void* rec;       // predicted records
+  int nj;
+  const double* ipos;    // ni x 3
+  const double* ivel;    // ni x 3
+  const double* iacc;    // ni x 3 (SIXTH)
+  const double* ih2;     // ni
+  const int* iid;        // ni
+  int ni;
+  double eps2;
+  double* part;          // part_fields x S x ni FP64 partial sums
+  double* part_ds2;      // S x ni  nearest-neighbour ds2 (working precision value)
+  int* part_addr;        // S x ni  nearest-neighbour address (INT_MAX: none)
+  int* ngb_cnt;          // ni exact neighbour counts (atomic)
+  int* ngb_list;         // ni x NGB_CAP ids (arbitrary order until finalized)
+  int ngb_on;            // neighbour search enabled (some h2 > 0)
+};
+
+// Launch geometry of one force call: each wave owns P i-particles and Q = 64/P
+// j-lanes per i-particle over one of S contiguous j-ranges of jchunk.
+struct ForceConfig { int P; int S; int jchunk; };
+
+struct FinalizeArgs {
+  const double* part; const double* part_ds2; const int* part_addr;
+  int S, ni;
+  const void* rec;       // to look up the neighbour's id
+  int* ngb_cnt; int* ngb_list;
+  double* out;           // ni x out_fields
+  int ngb_on;
+};
+
+struct PackArgs {
+  const int* ngb_cnt; const int* ngb_list; int ni;
+  int* offsets;          // ni + 1
+  int* packed;           // sum(min(cnt, NGB_CAP))
+};
+
+// Gathers the FP64 predicted state (pos, vel, [acc]) of all j in [0, n) into
+// a dense AoS buffer for read-back: 3 + 3 (+3) doubles plus mass per j.
+struct GatherArgs { StoreView st; int n; double* out; };
+
+int launch_scatter(hipStream_t s, int order, bool f64, const ScatterArgs& a);
+int launch_predict(hipStream_t s, int order, bool f64, const PredictArgs& a);
+int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const ForceConfig& c);
+int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a);
+int launch_pack(hipStream_t s, const PackArgs& a);
+int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a);
+
+// The P values with compiled force kernels.
+bool force_p_supported(int P);
+
+}  // namespace sapk

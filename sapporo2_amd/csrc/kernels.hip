@@ -1,0 +1,549 @@
+// kernels.hip -- gfx950 kernels of the direct-summation engine.
+//
+//   k_scatter     staged j-updates -> FP64 SoA store + predicted records
+//                 (reference dev_copy_particles, kernels.cu:245-301), fused with
+//                 the per-call reset of neighbour counters (dev_reset_buffers,
+//                 kernels.cu:1109-1144).
+//   k_predict     FP64 Horner predictor, HBM-bound (dev_predictor, kernels.cu:307-411).
+//   k_force       acc / pot / jerk [/ snap] + nearest neighbour + neighbour list
+//                 (dev_evaluate_gravity_*, kernels.cu:552-1041).  One wave owns P
+//                 i-particles held in registers and Q = 64/P j-lanes per
+//                 i-particle over one contiguous j-range (the "j-split").  With
+//                 P = 64 every lane of a wave reads the same j, so the j-record
+//                 is fetched once per wave through the scalar cache into SGPRs
+//                 and feeds the VALU directly.  Mixed precision: relative
+//                 positions by FP64 subtraction rounded to float, pair terms in
+//                 FP32, per-tile FP32 sums folded into FP64 accumulators.
+//   k_finalize    deterministic fixed-order reduction of the S partial sums,
+//                 nearest-neighbour selection and neighbour-list sort.
+//   k_pack_*      compacts the per-i neighbour lists for read-back.
+//
+// Built with -ffp-contract=off: every fused multiply-add below is explicit.
+#include "kernels.h"
+
+#include <climits>
+#include <cmath>
+#include <type_traits>
+
+namespace sapk {
+
+template <int ORDER, bool F64> struct RecOf;
+template <> struct RecOf<ORDER_H4, false> { using T = JRecM4; };
+template <> struct RecOf<ORDER_H6, false> { using T = JRecM6; };
+template <> struct RecOf<ORDER_H4, true> { using T = JRecD4; };
+template <> struct RecOf<ORDER_H6, true> { using T = JRecD6; };
+
+static_assert(sizeof(JRecM4) == 48, "record layout");
+static_assert(sizeof(JRecM6) == 64, "record layout");
+static_assert(sizeof(JRecD4) == 64, "record layout");
+static_assert(sizeof(JRecD6) == 96, "record layout");
+
+// Write the working-precision record of particle j from FP64 values.
+template <int ORDER, bool F64>
+__device__ __forceinline__ void put_rec(void* base, int j, double x, double y, double z, double m,
+                                        double vx, double vy, double vz, double ax, double ay,
+                                        double az, int id) {
+  using Rec = typename RecOf<ORDER, F64>::T;
+  using R = typename std::conditional<F64, double, float>::type;
+  Rec r{};
+  r.x = x; r.y = y; r.z = z;
+  r.m = (R)m; r.vx = (R)vx; r.vy = (R)vy; r.vz = (R)vz;
+  if constexpr (ORDER >= ORDER_H6) { r.ax = (R)ax; r.ay = (R)ay; r.az = (R)az; }
+  r.id = id;
+  reinterpret_cast<Rec*>(base)[j] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Scatter of staged updates (+ neighbour counter reset).
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  for (int k = tid; k < a.ni_reset; k += stride) a.ngb_cnt[k] = 0;
+  const size_t cap = (size_t)a.st.cap;
+  for (int k = tid; k < a.nupd; k += stride) {
+    const double* u = a.upd + (size_t)k * a.nf;
+    const int addr = a.upd_ai[2 * k];
+    const int id = a.upd_ai[2 * k + 1];
+    if (addr < 0 || addr >= a.st.cap) continue;  // host guarantees range
+    for (int f = 0; f < a.nf; ++f) a.st.raw[(size_t)f * cap + addr] = u[f];
+    a.st.id[addr] = id;
+    // predicted = raw for updated particles (kernels.cu:281-298)
+    double ax = 0, ay = 0, az = 0;
+    if constexpr (ORDER >= ORDER_H6) { ax = u[RF_AX]; ay = u[RF_AY]; az = u[RF_AZ]; }
+    put_rec<ORDER, F64>(a.st.rec, addr, u[RF_X], u[RF_Y], u[RF_Z], u[RF_M], u[RF_VX], u[RF_VY],
+                        u[RF_VZ], ax, ay, az, id);
+    if constexpr (!F64) {
+      a.st.pvel[addr] = u[RF_VX]; a.st.pvel[cap + addr] = u[RF_VY]; a.st.pvel[2 * cap + addr] = u[RF_VZ];
+      if constexpr (ORDER >= ORDER_H6) {
+        a.st.pacc[addr] = ax; a.st.pacc[cap + addr] = ay; a.st.pacc[2 * cap + addr] = az;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 predictor.  The expression is the reference's (kernels.cu:332-383),
+// evaluated without contraction, including the zero terms for FOURTH order.
+template <int ORDER>
+__device__ __forceinline__ void predict_one(const double* __restrict__ R, size_t cap, int j,
+                                            double ti, double p[3], double v[3], double ac[3]) {
+  const double dt = ti - R[RF_T * cap + j];
+  const double dt2 = (1. / 2.) * dt;
+  const double dt3 = (1. / 3.) * dt;
+  double dt4 = (1. / 4.) * dt;
+  double dt5 = (1. / 5.) * dt;
+  if constexpr (ORDER <= ORDER_H4) { dt4 = 0; dt5 = 0; }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double xs = R[(RF_X + d) * cap + j];
+    const double vs = R[(RF_VX + d) * cap + j];
+    const double as = R[(RF_AX + d) * cap + j];
+    const double js = R[(RF_JX + d) * cap + j];
+    double ss = 0.0, cs = 0.0;
+    if constexpr (ORDER >= ORDER_H6) { ss = R[(RF_SX + d) * cap + j]; cs = R[(RF_CX + d) * cap + j]; }
+    p[d] = xs + dt * (vs + dt2 * (as + dt3 * (js + dt4 * (ss + dt5 * (cs)))));
+    v[d] = vs + dt * (as + dt2 * (js + dt3 * (ss + dt4 * (cs))));
+    ac[d] = as;
+    if constexpr (ORDER >= ORDER_H6) ac[d] = as + dt * (js + dt2 * (ss + dt3 * (cs)));
+  }
+}
+
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_predict(PredictArgs a) {
+  const int j = a.j0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.j1) return;
+  const size_t cap = (size_t)a.st.cap;
+  double p[3], v[3], ac[3];
+  predict_one<ORDER>(a.st.raw, cap, j, a.ti, p, v, ac);
+  put_rec<ORDER, F64>(a.st.rec, j, p[0], p[1], p[2], a.st.raw[RF_M * cap + j], v[0], v[1], v[2],
+                      ac[0], ac[1], ac[2], a.st.id[j]);
+  if constexpr (!F64) {
+    a.st.pvel[j] = v[0]; a.st.pvel[cap + j] = v[1]; a.st.pvel[2 * cap + j] = v[2];
+    if constexpr (ORDER >= ORDER_H6) {
+      a.st.pacc[j] = ac[0]; a.st.pacc[cap + j] = ac[1]; a.st.pacc[2 * cap + j] = ac[2];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Force kernel.
+template <typename R> __device__ __forceinline__ R fmaR(R a, R b, R c);
+template <> __device__ __forceinline__ float fmaR<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fmaR<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// 1/sqrt: v_rsq_f32 (1 ulp) in FP32; in FP64 v_rsq_f64 plus one cubic
+// correction step, full double precision for the positive normal operands
+// used here (r2 >= eps2 > 0; other cases are masked by the caller).
+template <typename R> __device__ __forceinline__ R rsq(R x);
+template <> __device__ __forceinline__ float rsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> __device__ __forceinline__ double rsq<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = __builtin_fma(-x * y, y, 1.0);
+  return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
+}
+
+template <typename R> __device__ __forceinline__ R to_rel(double xj, double xi) { return (R)(xj - xi); }
+
+// Per-lane state of one i-particle.
+template <int ORDER, typename R>
+struct IState {
+  double x, y, z;
+  R vx, vy, vz, ax, ay, az, h2;
+  int id, i;
+};
+
+constexpr int NF4 = 7, NF6 = 10;
+template <int ORDER> constexpr int nf() { return ORDER >= ORDER_H6 ? NF6 : NF4; }
+
+// One pair interaction (kernels.cu:565-667) accumulated into t[].
+template <int ORDER, bool NGB, typename R, typename Rec>
+__device__ __forceinline__ void pair(const Rec& r, const int j, const bool ok, const IState<ORDER, R>& s,
+                                     const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
+                                     int* __restrict__ ngb_cnt, int* __restrict__ ngb_list) {
+  const R dx = to_rel<R>(r.x, s.x);
+  const R dy = to_rel<R>(r.y, s.y);
+  const R dz = to_rel<R>(r.z, s.z);
+  const R ds2 = fmaR<R>(dz, dz, fmaR<R>(dy, dy, dx * dx));
+  const bool use = ok && (r.id != s.id);
+  const R inv0 = rsq<R>(ds2 + eps2);
+  const R inv = use ? inv0 : R(0);
+  const R invr2 = inv * inv;
+  const R mr1 = r.m * inv;
+  const R mr3 = mr1 * invr2;
+  t[0] = fmaR<R>(mr3, dx, t[0]);
+  t[1] = fmaR<R>(mr3, dy, t[1]);
+  t[2] = fmaR<R>(mr3, dz, t[2]);
+  t[3] = t[3] - mr1;
+  const R dvx = r.vx - s.vx;
+  const R dvy = r.vy - s.vy;
+  const R dvz = r.vz - s.vz;
+  const R rv = fmaR<R>(dz, dvz, fmaR<R>(dy, dvy, dx * dvx));
+  if constexpr (ORDER == ORDER_H4) {
+    // jerk += m/r^3 dv - 3 (m/r^3)(dr.dv)/r^2 dr   (kernels.cu:633-636)
+    const R q = (rv * invr2) * R(-3);
+    t[4] = fmaR<R>(mr3, fmaR<R>(q, dx, dvx), t[4]);
+    t[5] = fmaR<R>(mr3, fmaR<R>(q, dy, dvy), t[5]);
+    t[6] = fmaR<R>(mr3, fmaR<R>(q, dz, dvz), t[6]);
+  } else {
+    // kernels.cu:642-667: alpha = dr.dv/r^2, beta = (dv^2 + dr.da)/r^2 + alpha^2
+    const R dax = r.ax - s.ax;
+    const R day = r.ay - s.ay;
+    const R daz = r.az - s.az;
+    const R v2 = fmaR<R>(dvz, dvz, fmaR<R>(dvy, dvy, dvx * dvx));
+    const R ra = fmaR<R>(dz, daz, fmaR<R>(dy, day, dx * dax));
+    const R alpha = rv * invr2;
+    const R beta = fmaR<R>(v2 + ra, invr2, alpha * alpha);
+    const R m3a = alpha * R(-3);
+    // jerk_ij = m/r^3 (dv - 3 alpha dr)
+    const R jx = mr3 * fmaR<R>(m3a, dx, dvx);
+    const R jy = mr3 * fmaR<R>(m3a, dy, dvy);
+    const R jz = mr3 * fmaR<R>(m3a, dz, dvz);
+    // snap_ij = m/r^3 (da - 3 beta dr) - 6 alpha jerk_ij
+    const R m3b = beta * R(-3);
+    const R m6a = alpha * R(-6);
+    t[7] = fmaR<R>(m6a, jx, fmaR<R>(mr3, fmaR<R>(m3b, dx, dax), t[7]));
+    t[8] = fmaR<R>(m6a, jy, fmaR<R>(mr3, fmaR<R>(m3b, dy, day), t[8]));
+    t[9] = fmaR<R>(m6a, jz, fmaR<R>(mr3, fmaR<R>(m3b, dz, daz), t[9]));
+    t[4] = t[4] + jx;
+    t[5] = t[5] + jy;
+    t[6] = t[6] + jz;
+  }
+  // nearest neighbour: strict <, first address wins inside a lane (kernels.cu:539-546)
+  const bool better = use && (ds2 < dsmin);
+  dsmin = better ? ds2 : dsmin;
+  nnaddr = better ? j : nnaddr;
+  // neighbour list: h2_i > ds2 (kernels.cu:584); rare, so a global atomic slot
+  if constexpr (NGB) {
+    if (use && (s.h2 > ds2)) {
+      const int k = atomicAdd(&ngb_cnt[s.i], 1);
+      if (k < NGB_CAP) ngb_list[(size_t)s.i * NGB_CAP + k] = r.id;
+    }
+  }
+}
+
+constexpr int KT = 64;  // interactions per lane between FP64 folds
+
+template <int ORDER, bool F64, int P, bool NGB>
+__global__ __launch_bounds__(256) void k_force(
+    const typename RecOf<ORDER, F64>::T* __restrict__ rec, const int nj,
+    const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const double* __restrict__ iacc, const double* __restrict__ ih2,
+    const int* __restrict__ iid, const int ni, const double eps2d,
+    double* __restrict__ part, double* __restrict__ part_ds2, int* __restrict__ part_addr,
+    int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, const int S, const int jchunk) {
+  using R = typename std::conditional<F64, double, float>::type;
+  constexpr int Q = 64 / P;
+  constexpr int NF = nf<ORDER>();
+  const int lane = threadIdx.x & 63;
+  // wave-uniform by construction; readfirstlane lets the compiler keep the
+  // j-loop bookkeeping and (for P = 64) the j-record loads in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int G = (ni + P - 1) / P;
+  if (wave >= G * S) return;  // whole wave
+  const int g = wave / S;
+  const int sidx = wave - g * S;
+  const int islot = (Q == 1) ? lane : (lane % P);
+  const int jsub = (Q == 1) ? 0 : (lane / P);
+  const int i = g * P + islot;
+  const bool ivalid = i < ni;
+  const int il = ivalid ? i : ni - 1;
+
+  IState<ORDER, R> st;
+  st.x = ipos[3 * il]; st.y = ipos[3 * il + 1]; st.z = ipos[3 * il + 2];
+  st.vx = (R)ivel[3 * il]; st.vy = (R)ivel[3 * il + 1]; st.vz = (R)ivel[3 * il + 2];
+  st.ax = st.ay = st.az = R(0);
+  if constexpr (ORDER >= ORDER_H6) {
+    st.ax = (R)iacc[3 * il]; st.ay = (R)iacc[3 * il + 1]; st.az = (R)iacc[3 * il + 2];
+  }
+  st.h2 = (NGB && ivalid) ? (R)ih2[il] : R(-1);
+  st.id = iid[il];
+  st.i = il;
+  const R eps2 = (R)eps2d;
+
+  double T[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) T[f] = 0.0;
+  R dsmin = R(INFINITY);
+  int nnaddr = INT_MAX;
+
+  const int jb = sidx * jchunk;
+  const int je = min(nj, jb + jchunk);
+  const int nk = (je > jb) ? (je - jb + Q - 1) / Q : 0;  // iterations of the lane with jsub 0
+  for (int k0 = 0; k0 < nk; k0 += KT) {
+    R t[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) t[f] = R(0);
+    const int kn = min(KT, nk - k0);
+    const int jt = jb + k0 * Q + jsub;
+    if (kn == KT && jb + (k0 + KT) * Q <= je) {
+#pragma unroll 4
+      for (int k = 0; k < KT; ++k) {
+        const int j = jt + k * Q;
+        pair<ORDER, NGB>(rec[j], j, true, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+      }
+    } else {
+      for (int k = 0; k < kn; ++k) {
+        const int j = jt + k * Q;
+        const bool ok = j < je;
+        pair<ORDER, NGB>(rec[ok ? j : jb], j, ok, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) T[f] += (double)t[f];
+  }
+
+  double dsd = (double)dsmin;
+  if constexpr (Q > 1) {
+#pragma unroll
+    for (int m = P; m < 64; m <<= 1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) T[f] += __shfl_xor(T[f], m);
+      const double od = __shfl_xor(dsd, m);
+      const int oa = __shfl_xor(nnaddr, m);
+      const bool take = (od < dsd) || (od == dsd && oa < nnaddr);
+      dsd = take ? od : dsd;
+      nnaddr = take ? oa : nnaddr;
+    }
+  }
+  if (ivalid && jsub == 0) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) part[((size_t)f * S + sidx) * ni + i] = T[f];
+    part_ds2[(size_t)sidx * ni + i] = dsd;
+    part_addr[(size_t)sidx * ni + i] = nnaddr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Finalize: block of 4 waves handles 4 i-particles (one per wave).
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) {
+  using Rec = typename RecOf<ORDER, F64>::T;
+  constexpr int NF = nf<ORDER>();
+  constexpr int NO = out_fields(ORDER);
+  __shared__ int lst[4][NGB_CAP];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 4 + w;
+  const bool active = i < a.ni;
+  int c = 0;
+  if (active) {
+    double T[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) T[f] = 0.0;
+    double ds = INFINITY;
+    int ad = INT_MAX;
+    for (int s = lane; s < a.S; s += 64) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * a.ni + i];
+      const double d = a.part_ds2[(size_t)s * a.ni + i];
+      const int ad2 = a.part_addr[(size_t)s * a.ni + i];
+      const bool take = (d < ds) || (d == ds && ad2 < ad);
+      ds = take ? d : ds;
+      ad = take ? ad2 : ad;
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) T[f] += __shfl_xor(T[f], m);
+      const double od = __shfl_xor(ds, m);
+      const int oa = __shfl_xor(ad, m);
+      const bool take = (od < ds) || (od == ds && oa < ad);
+      ds = take ? od : ds;
+      ad = take ? oa : ad;
+    }
+    const int cnt = a.ngb_on ? a.ngb_cnt[i] : 0;
+    if (lane == 0) {
+      double* o = a.out + (size_t)i * NO;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) o[f] = T[f];
+      const bool has = ad != INT_MAX;
+      // the device returns ds2 rounded to float (kernels.cu:999-1002)
+      o[NF] = has ? (double)(float)ds : INFINITY;
+      o[NF + 1] = has ? (double)reinterpret_cast<const Rec*>(a.rec)[ad].id : -1.0;
+      o[NF + 2] = (double)cnt;
+    }
+    c = min(cnt, NGB_CAP);
+    if (c > 1)
+      for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
+  }
+  __syncthreads();
+  if (active && c > 1) {
+    // rank sort (ascending ids, stable for equal ids)
+    for (int k = lane; k < c; k += 64) {
+      const int v = lst[w][k];
+      int rank = 0;
+      for (int q = 0; q < c; ++q) {
+        const int u = lst[w][q];
+        rank += (u < v) || (u == v && q < k);
+      }
+      a.ngb_list[(size_t)i * NGB_CAP + rank] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-list packing: offsets by a single-block scan, then a copy.
+__global__ __launch_bounds__(1024) void k_pack_scan(PackArgs a) {
+  __shared__ int sm[1024];
+  const int t = threadIdx.x;
+  const int per = (a.ni + 1023) / 1024;
+  const int b = t * per, e = min(a.ni, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += min(a.ngb_cnt[i], NGB_CAP);
+  sm[t] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int v = (t >= d) ? sm[t - d] : 0;
+    __syncthreads();
+    sm[t] += v;
+    __syncthreads();
+  }
+  int off = sm[t] - s;
+  for (int i = b; i < e; ++i) {
+    a.offsets[i] = off;
+    off += min(a.ngb_cnt[i], NGB_CAP);
+  }
+  if (t == 1023) a.offsets[a.ni] = sm[1023];
+}
+
+__global__ __launch_bounds__(256) void k_pack_copy(PackArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= a.ni) return;
+  const int c = min(a.ngb_cnt[i], NGB_CAP);
+  const int o = a.offsets[i];
+  for (int k = lane; k < c; k += 64) a.packed[o + k] = a.ngb_list[(size_t)i * NGB_CAP + k];
+}
+
+// ---------------------------------------------------------------------------
+// Predicted-state gather for read-back: per j, 10 doubles
+// (x, y, z, m, vx, vy, vz, ax, ay, az) in FP64, plus the id in the m slot's
+// companion array handled by the host.
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_gather_pred(GatherArgs a) {
+  using Rec = typename RecOf<ORDER, F64>::T;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  const Rec r = reinterpret_cast<const Rec*>(a.st.rec)[j];
+  const size_t cap = (size_t)a.st.cap;
+  double* o = a.out + (size_t)j * 10;
+  o[0] = r.x; o[1] = r.y; o[2] = r.z;
+  o[3] = a.st.raw[RF_M * cap + j];
+  if constexpr (F64) {
+    o[4] = r.vx; o[5] = r.vy; o[6] = r.vz;
+    if constexpr (ORDER >= ORDER_H6) { o[7] = r.ax; o[8] = r.ay; o[9] = r.az; }
+    else { o[7] = o[8] = o[9] = 0.0; }
+  } else {
+    o[4] = a.st.pvel[j]; o[5] = a.st.pvel[cap + j]; o[6] = a.st.pvel[2 * cap + j];
+    if constexpr (ORDER >= ORDER_H6) { o[7] = a.st.pacc[j]; o[8] = a.st.pacc[cap + j]; o[9] = a.st.pacc[2 * cap + j]; }
+    else { o[7] = o[8] = o[9] = 0.0; }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+static inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+static inline int hip_status() { return hipGetLastError() == hipSuccess ? 0 : -1; }
+
+// Calls f(order constant, precision constant) for the runtime (order, f64).
+template <typename F>
+static void dispatch_op(int order, bool f64, F&& f) {
+  using H4 = std::integral_constant<int, ORDER_H4>;
+  using H6 = std::integral_constant<int, ORDER_H6>;
+  if (order >= ORDER_H6) {
+    if (f64) f(H6{}, std::true_type{}); else f(H6{}, std::false_type{});
+  } else {
+    if (f64) f(H4{}, std::true_type{}); else f(H4{}, std::false_type{});
+  }
+}
+
+int launch_scatter(hipStream_t s, int order, bool f64, const ScatterArgs& a) {
+  const long n = a.nupd > a.ni_reset ? a.nupd : a.ni_reset;
+  if (n <= 0) return 0;
+  int nb = blocks_for(n, 256);
+  if (nb > 4096) nb = 4096;
+  dispatch_op(order, f64, [&](auto OR, auto FP) {
+    hipLaunchKernelGGL((k_scatter<decltype(OR)::value, decltype(FP)::value>), dim3(nb), dim3(256), 0, s, a);
+  });
+  return hip_status();
+}
+
+int launch_predict(hipStream_t s, int order, bool f64, const PredictArgs& a) {
+  if (a.j1 <= a.j0) return 0;
+  const int nb = blocks_for(a.j1 - a.j0, 256);
+  dispatch_op(order, f64, [&](auto OR, auto FP) {
+    hipLaunchKernelGGL((k_predict<decltype(OR)::value, decltype(FP)::value>), dim3(nb), dim3(256), 0, s, a);
+  });
+  return hip_status();
+}
+
+bool force_p_supported(int P) {
+  return P == 64 || P == 32 || P == 16 || P == 8 || P == 4 || P == 2 || P == 1;
+}
+
+template <int OR, bool FP, int P>
+static void launch_force_p(hipStream_t s, const ForceArgs& a, const ForceConfig& c) {
+  using Rec = typename RecOf<OR, FP>::T;
+  const long G = (a.ni + P - 1) / P;
+  const int nb = blocks_for(G * c.S, 4);
+  const Rec* rec = reinterpret_cast<const Rec*>(a.rec);
+  if (a.ngb_on)
+    hipLaunchKernelGGL((k_force<OR, FP, P, true>), dim3(nb), dim3(256), 0, s, rec, a.nj, a.ipos, a.ivel,
+                       a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, a.part_addr, a.ngb_cnt,
+                       a.ngb_list, c.S, c.jchunk);
+  else
+    hipLaunchKernelGGL((k_force<OR, FP, P, false>), dim3(nb), dim3(256), 0, s, rec, a.nj, a.ipos, a.ivel,
+                       a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, a.part_addr, a.ngb_cnt,
+                       a.ngb_list, c.S, c.jchunk);
+}
+
+template <int OR, bool FP>
+static int launch_force_t(hipStream_t s, const ForceArgs& a, const ForceConfig& c) {
+  switch (c.P) {
+    case 64: launch_force_p<OR, FP, 64>(s, a, c); break;
+    case 32: launch_force_p<OR, FP, 32>(s, a, c); break;
+    case 16: launch_force_p<OR, FP, 16>(s, a, c); break;
+    case 8: launch_force_p<OR, FP, 8>(s, a, c); break;
+    case 4: launch_force_p<OR, FP, 4>(s, a, c); break;
+    case 2: launch_force_p<OR, FP, 2>(s, a, c); break;
+    case 1: launch_force_p<OR, FP, 1>(s, a, c); break;
+    default: return -2;
+  }
+  return hip_status();
+}
+
+int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const ForceConfig& c) {
+  if (a.ni <= 0 || a.nj <= 0) return 0;
+  if (c.S <= 0 || c.jchunk <= 0 || (long)c.S * c.jchunk < a.nj) return -3;
+  int rc = 0;
+  dispatch_op(order, f64, [&](auto OR, auto FP) { rc = launch_force_t<decltype(OR)::value, decltype(FP)::value>(s, a, c); });
+  return rc;
+}
+
+int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a) {
+  if (a.ni <= 0) return 0;
+  const int nb = blocks_for(a.ni, 4);
+  dispatch_op(order, f64, [&](auto OR, auto FP) {
+    hipLaunchKernelGGL((k_finalize<decltype(OR)::value, decltype(FP)::value>), dim3(nb), dim3(256), 0, s, a);
+  });
+  return hip_status();
+}
+
+int launch_pack(hipStream_t s, const PackArgs& a) {
+  if (a.ni <= 0) return 0;
+  hipLaunchKernelGGL(k_pack_scan, dim3(1), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(k_pack_copy, dim3(blocks_for(a.ni, 4)), dim3(256), 0, s, a);
+  return hip_status();
+}
+
+int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a) {
+  if (a.n <= 0) return 0;
+  const int nb = blocks_for(a.n, 256);
+  dispatch_op(order, f64, [&](auto OR, auto FP) {
+    hipLaunchKernelGGL((k_gather_pred<decltype(OR)::value, decltype(FP)::value>), dim3(nb), dim3(256), 0, s, a);
+  });
+  return hip_status();
+}
+
+}  // namespace sapk

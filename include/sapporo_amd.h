@@ -42,6 +42,14 @@ int sapamd_set_j_particle(sapamd_engine *e, int address, int id, double tj, doub
                           double mass, const double acc[3], const double jrk[3],
                           const double vel[3], const double pos[3], const double snp[3],
                           const double crk[3], double eps);
+/* Array form of sapamd_set_j_particle for n particles (no reference
+ * counterpart; the reference takes one particle per call).  Vectors are
+ * n x 3; tj, dtj, snp, crk and eps may be NULL (zeros). */
+int sapamd_set_j_particles(sapamd_engine *e, int n, const int *address, const int *id,
+                           const double *tj, const double *dtj, const double *mass,
+                           const double *acc, const double *jrk, const double *vel,
+                           const double *pos, const double *snp, const double *crk,
+                           const double *eps);
 /* sapporohostclass.h:141-147  startGravCalc; xi, vi, ai are ni x 3; ai may be
  * NULL for FOURTH; h2 may be NULL (no neighbour lists). */
 int sapamd_start_grav_calc(sapamd_engine *e, int nj, int ni, const int *id, const double *xi,
@@ -71,6 +79,9 @@ int sapamd_set_force_override(sapamd_engine *e, int P, int S);
 int sapamd_set_timing(sapamd_engine *e, int on);
 float sapamd_last_force_ms(sapamd_engine *e);
 float sapamd_last_predict_ms(sapamd_engine *e);
+/* device time of the last force call from the first kernel after its input
+ * copies to the end of the finalize kernel (copies excluded) */
+float sapamd_last_device_ms(sapamd_engine *e);
 int sapamd_last_config(sapamd_engine *e, int *P, int *S, int *jchunk);
 int sapamd_num_cus(sapamd_engine *e);
 int sapamd_synchronize(sapamd_engine *e);

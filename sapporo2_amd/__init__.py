@@ -49,7 +49,7 @@ def build(verbose=False):
 ENGINE_SYMBOLS = [
     "sapamd_create", "sapamd_destroy", "sapamd_open", "sapamd_close", "sapamd_get_n_pipes",
     "sapamd_set_time", "sapamd_set_no_time", "sapamd_set_j_particle", "sapamd_start_grav_calc",
-    "sapamd_get_grav_results", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
+    "sapamd_get_grav_results", "sapamd_set_j_particles", "sapamd_last_device_ms", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
     "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_timing",
     "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_num_cus",
@@ -75,6 +75,10 @@ def lib():
         L.sapamd_set_j_particle.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp,
                                             _dp, _dp, ctypes.c_double]
+        L.sapamd_set_j_particles.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp,
+                                             _dp, _dp, _dp, _dp, _dp]
+        L.sapamd_last_device_ms.argtypes = [_vp]
+        L.sapamd_last_device_ms.restype = ctypes.c_float
         L.sapamd_start_grav_calc.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp,
                                              ctypes.c_double, _dp]
         L.sapamd_get_grav_results.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
@@ -183,26 +187,18 @@ class Engine:
             *[_pd(v) for v in a], float(eps)), "set_j_particle")
 
     def set_j_particles(self, addresses, ids, mass, pos, vel, acc=None, jrk=None, snp=None,
-                        crk=None, tj=0.0, eps=0.0):
-        """Vector convenience: one set_j_particle call per particle."""
+                        crk=None, tj=0.0, eps=None):
+        """Array form (one C call): same per-particle semantics as set_j_particle."""
         n = len(addresses)
-        pos = _d(pos, (n, 3)); vel = _d(vel, (n, 3))
-        z = np.zeros((n, 3))
-        acc = z if acc is None else _d(acc, (n, 3))
-        jrk = z if jrk is None else _d(jrk, (n, 3))
-        snp = None if snp is None else _d(snp, (n, 3))
-        crk = None if crk is None else _d(crk, (n, 3))
-        tj = np.broadcast_to(np.asarray(tj, np.float64), (n,))
-        mass = np.asarray(mass, np.float64)
-        f = self._L.sapamd_set_j_particle
-        h = self._h
-        for k in range(n):
-            rc = f(h, int(addresses[k]), int(ids[k]), float(tj[k]), 0.0, float(mass[k]),
-                   _pd(acc[k]), _pd(jrk[k]), _pd(vel[k]), _pd(pos[k]),
-                   None if snp is None else _pd(snp[k]), None if crk is None else _pd(crk[k]),
-                   float(eps))
-            if rc < 0:
-                self._check(rc, "set_j_particle")
+        addr = np.ascontiguousarray(addresses, np.int32)
+        idv = np.ascontiguousarray(ids, np.int32)
+        mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float64), (n,)))
+        tjv = np.ascontiguousarray(np.broadcast_to(np.asarray(tj, np.float64), (n,)))
+        arrs = [None if a is None else _d(a, (n, 3)) for a in (acc, jrk, vel, pos, snp, crk)]
+        epsv = None if eps is None else np.ascontiguousarray(np.broadcast_to(np.asarray(eps, np.float64), (n,)))
+        return self._check(self._L.sapamd_set_j_particles(
+            self._h, n, _pi(addr), _pi(idv), _pd(tjv), None, _pd(mass), *[_pd(a) for a in arrs],
+            _pd(epsv)), "set_j_particles")
 
     def start_grav_calc(self, nj, ids, xi, vi, eps2, h2=None, ai=None):
         ni = len(ids)
@@ -282,6 +278,9 @@ class Engine:
 
     def last_force_ms(self):
         return float(self._L.sapamd_last_force_ms(self._h))
+
+    def last_device_ms(self):
+        return float(self._L.sapamd_last_device_ms(self._h))
 
     def last_predict_ms(self):
         return float(self._L.sapamd_last_predict_ms(self._h))

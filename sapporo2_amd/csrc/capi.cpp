@@ -31,6 +31,13 @@ int sapamd_set_j_particle(sapamd_engine* e, int address, int id, double tj, doub
   return e ? e->eng.set_j_particle(address, id, tj, dtj, mass, acc, jrk, vel, pos, snp, crk, eps) : -1;
 }
 
+int sapamd_set_j_particles(sapamd_engine* e, int n, const int* address, const int* id, const double* tj,
+                           const double* dtj, const double* mass, const double* acc, const double* jrk,
+                           const double* vel, const double* pos, const double* snp, const double* crk,
+                           const double* eps) {
+  return e ? e->eng.set_j_particles(n, address, id, tj, dtj, mass, acc, jrk, vel, pos, snp, crk, eps) : -1;
+}
+
 int sapamd_start_grav_calc(sapamd_engine* e, int nj, int ni, const int* id, const double* xi,
                            const double* vi, const double* ai, double eps2, const double* h2) {
   return e ? e->eng.start_grav_calc(nj, ni, id, reinterpret_cast<V3>(xi), reinterpret_cast<V3>(vi),
@@ -76,6 +83,7 @@ int sapamd_set_timing(sapamd_engine* e, int on) {
 }
 float sapamd_last_force_ms(sapamd_engine* e) { return e ? e->eng.last_force_ms() : 0.f; }
 float sapamd_last_predict_ms(sapamd_engine* e) { return e ? e->eng.last_predict_ms() : 0.f; }
+float sapamd_last_device_ms(sapamd_engine* e) { return e ? e->eng.last_device_ms() : 0.f; }
 int sapamd_last_config(sapamd_engine* e, int* P, int* S, int* jchunk) {
   if (!e) return -1;
   const sapk::ForceConfig c = e->eng.last_config();

@@ -81,6 +81,8 @@ int Engine::open(int device, int order, int precision) {
   cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   HIPCHK(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_d0_), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_d1_), "hipEventCreate");
   HIPCHK(hipEventCreate(&ev_f0_), "hipEventCreate");
   HIPCHK(hipEventCreate(&ev_f1_), "hipEventCreate");
   HIPCHK(hipEventCreate(&ev_p0_), "hipEventCreate");
@@ -135,7 +137,7 @@ int Engine::close() {
   dfree(cnt_d_); dfree(list_d_); dfree(off_d_); dfree(packed_d_);
   hfree(off_h_); hfree(packed_h_);
   dfree(gather_d_); gather_cap_ = 0;
-  for (hipEvent_t* e : {&ev_done_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
+  for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
   }
@@ -289,44 +291,69 @@ int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass,
   return 0;
 }
 
-// Reference send_j_particles_to_device + copyJInDev (sapporohostclass.cpp:889-923,
-// 1204-1248), fused with the neighbour-counter reset of dev_reset_buffers.
-int Engine::flush_updates(int ni_reset) {
-  const int n = nstaged_;
-  const int nf = raw_fields(order_);
-  if (n > 0) {
-    if (ensure_store(max_addr_ + 1) != 0) return -1;
-    if (upd_cap_d_ < n) {
-      dfree(upd_d_);
-      dfree(upd_ai_d_);
-      const int nc = std::max(n, upd_cap_d_ * 2);
-      HIPCHK(dmalloc(&upd_d_, (size_t)nc * nf), "device staging");
-      HIPCHK(dmalloc(&upd_ai_d_, (size_t)nc * 2), "device staging ids");
-      upd_cap_d_ = nc;
-    }
-    Stage& sg = stage_[cur_];
-    HIPCHK(hipMemcpyAsync(upd_d_, sg.d, sizeof(double) * (size_t)n * nf, hipMemcpyHostToDevice, stream_),
-           "update upload");
-    HIPCHK(hipMemcpyAsync(upd_ai_d_, sg.ai, sizeof(int) * (size_t)n * 2, hipMemcpyHostToDevice, stream_),
-           "update upload");
-    HIPCHK(hipEventRecord(sg.ev, stream_), "event");
-    sg.busy = true;
-    for (int k = 0; k < n; ++k) slot_of_[sg.ai[2 * k]] = -1;
-    nstaged_ = 0;
-    cur_ ^= 1;
-  }
-  if (n > 0 || ni_reset > 0) {
-    ScatterArgs a{};
-    a.st = st_;
-    a.upd = upd_d_;
-    a.upd_ai = upd_ai_d_;
-    a.nupd = n;
-    a.nf = nf;
-    a.ngb_cnt = cnt_d_;
-    a.ni_reset = ni_reset;
-    if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
+int Engine::set_j_particles(int n, const int* addr, const int* id, const double* tj, const double* dtj,
+                            const double* mass, const double* a, const double* jrk, const double* v,
+                            const double* x, const double* snp, const double* crk, const double* eps) {
+  if (n < 0 || !addr || !id || !mass || !x) return fail("set_j_particles: bad arguments");
+  for (int k = 0; k < n; ++k) {
+    const size_t o = 3 * (size_t)k;
+    const int rc = set_j_particle(addr[k], id[k], tj ? tj[k] : 0.0, dtj ? dtj[k] : 0.0, mass[k],
+                                  a ? a + o : nullptr, jrk ? jrk + o : nullptr, v ? v + o : nullptr, x + o,
+                                  snp ? snp + o : nullptr, crk ? crk + o : nullptr, eps ? eps[k] : 0.0);
+    if (rc != 0) return rc;
   }
   return 0;
+}
+
+// Reference send_j_particles_to_device + copyJInDev (sapporohostclass.cpp:889-923,
+// 1204-1248), fused with the neighbour-counter reset of dev_reset_buffers.
+// upload_updates issues the host->device copy of the staged updates; the
+// scatter kernel is launched separately so callers can bracket device work.
+int Engine::upload_updates(int* nupd) {
+  const int n = nstaged_;
+  const int nf = raw_fields(order_);
+  *nupd = n;
+  if (n <= 0) return 0;
+  if (ensure_store(max_addr_ + 1) != 0) return -1;
+  if (upd_cap_d_ < n) {
+    dfree(upd_d_);
+    dfree(upd_ai_d_);
+    const int nc = std::max(n, upd_cap_d_ * 2);
+    HIPCHK(dmalloc(&upd_d_, (size_t)nc * nf), "device staging");
+    HIPCHK(dmalloc(&upd_ai_d_, (size_t)nc * 2), "device staging ids");
+    upd_cap_d_ = nc;
+  }
+  Stage& sg = stage_[cur_];
+  HIPCHK(hipMemcpyAsync(upd_d_, sg.d, sizeof(double) * (size_t)n * nf, hipMemcpyHostToDevice, stream_),
+         "update upload");
+  HIPCHK(hipMemcpyAsync(upd_ai_d_, sg.ai, sizeof(int) * (size_t)n * 2, hipMemcpyHostToDevice, stream_),
+         "update upload");
+  HIPCHK(hipEventRecord(sg.ev, stream_), "event");
+  sg.busy = true;
+  for (int k = 0; k < n; ++k) slot_of_[sg.ai[2 * k]] = -1;
+  nstaged_ = 0;
+  cur_ ^= 1;
+  return 0;
+}
+
+int Engine::scatter_updates(int n, int ni_reset) {
+  if (n <= 0 && ni_reset <= 0) return 0;
+  ScatterArgs a{};
+  a.st = st_;
+  a.upd = upd_d_;
+  a.upd_ai = upd_ai_d_;
+  a.nupd = n;
+  a.nf = raw_fields(order_);
+  a.ngb_cnt = cnt_d_;
+  a.ni_reset = ni_reset;
+  if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
+  return 0;
+}
+
+int Engine::flush_updates(int ni_reset) {
+  int n = 0;
+  if (upload_updates(&n) != 0) return -1;
+  return scatter_updates(n, ni_reset);
 }
 
 int Engine::predict_range(int j0, int j1) {
@@ -417,7 +444,11 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
   HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
 
-  if (flush_updates(ngb_on ? ni : 0) != 0) return -1;
+  int nupd = 0;
+  if (upload_updates(&nupd) != 0) return -1;
+  // device window: everything after the host->device copies
+  if (timing_) HIPCHK(hipEventRecord(ev_d0_, stream_), "event");
+  if (scatter_updates(nupd, ngb_on ? ni : 0) != 0) return -1;
   if (predict_) {
     if (predict_range(0, nj) != 0) return -1;
   } else {
@@ -460,6 +491,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   fz.out = out_d_;
   fz.ngb_on = fa.ngb_on;
   if (launch_finalize(stream_, order_, f64_, fz) != 0) return fail("finalize launch", hipGetLastError());
+  if (timing_) HIPCHK(hipEventRecord(ev_d1_, stream_), "event");
   HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
                         hipMemcpyDeviceToHost, stream_), "result download");
   HIPCHK(hipEventRecord(ev_done_, stream_), "event");
@@ -487,6 +519,14 @@ float Engine::last_force_ms() {
   (void)hipEventSynchronize(ev_f1_);
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, ev_f0_, ev_f1_) != hipSuccess) return 0.f;
+  return ms;
+}
+
+float Engine::last_device_ms() {
+  if (!timing_ || !have_results_) return 0.f;
+  (void)hipEventSynchronize(ev_d1_);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ev_d0_, ev_d1_) != hipSuccess) return 0.f;
   return ms;
 }
 

@@ -43,6 +43,11 @@ class Engine {
   int set_j_particle(int addr, int id, double tj, double dtj, double mass, const double a[3],
                      const double jrk[3], const double v[3], const double x[3], const double snp[3],
                      const double crk[3], double eps);
+  // Array form of set_j_particle for n particles (vectors are n x 3 row-major;
+  // tj, dtj, eps, snp, crk may be NULL).  Same per-particle semantics.
+  int set_j_particles(int n, const int* addr, const int* id, const double* tj, const double* dtj,
+                      const double* mass, const double* a, const double* jrk, const double* v,
+                      const double* x, const double* snp, const double* crk, const double* eps);
   int start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3], const double (*vi)[3],
                       const double (*ai)[3], double eps2, const double* h2);
   int get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3], double (*snp)[3],
@@ -62,6 +67,7 @@ class Engine {
   void set_timing(bool on) { timing_ = on; }
   float last_force_ms();          // force kernel time of the last call (timing on)
   float last_predict_ms();        // predictor time of the last call (timing on, 0 if none)
+  float last_device_ms();         // scatter..finalize device time of the last call (timing on)
   sapk::ForceConfig last_config() const { return last_cfg_; }
   int device() const { return device_; }
   int num_cus() const { return cus_; }
@@ -74,6 +80,8 @@ class Engine {
   int ensure_store(int need);
   int ensure_partials(size_t entries);
   int flush_updates(int ni_reset);
+  int upload_updates(int* nupd);
+  int scatter_updates(int n, int ni_reset);
   int predict_range(int j0, int j1);
   int wait_results();
   int fetch_j_host(int nj);
@@ -118,7 +126,7 @@ class Engine {
   int* cnt_d_ = nullptr; int* list_d_ = nullptr;
   int* off_d_ = nullptr; int* packed_d_ = nullptr;
   int* off_h_ = nullptr; int* packed_h_ = nullptr;
-  hipEvent_t ev_done_ = nullptr, ev_f0_ = nullptr, ev_f1_ = nullptr, ev_p0_ = nullptr, ev_p1_ = nullptr;
+  hipEvent_t ev_done_ = nullptr, ev_d0_ = nullptr, ev_d1_ = nullptr, ev_f0_ = nullptr, ev_f1_ = nullptr, ev_p0_ = nullptr, ev_p1_ = nullptr;
   bool inflight_ = false, have_results_ = false, ngb_fetched_ = false, ngb_on_last_ = false;
   bool timed_predict_ = false;
   int ni_last_ = 0, nj_last_ = 0;

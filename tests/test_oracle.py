@@ -1,0 +1,383 @@
+"""Pins for the CPU oracle (oracle/oracle.c), independent of the oracle's own code.
+
+The reference ships no golden vectors (its tests print, assert nothing), so each
+oracle function is pinned by what the mathematics fixes:
+  predictor      exact rational evaluation of the Taylor polynomial; dt = 0 identity;
+                 exact dyadic cases (bit-exact)
+  forces         two-body closed forms; jerk and snap equal the analytic time
+                 derivatives of the acceleration (Richardson finite differences);
+                 high-precision Decimal brute force on tiny N; symmetric
+                 configurations; Newton's third law; the Plummer potential
+  NN / lists     brute force with exact IEEE rounding emulation (Fractions) of the
+                 stated distance contract; lattice ties; ds2 == h2 boundary;
+                 255/256/257-neighbour overflow; the 1e10 read-back threshold
+"""
+from __future__ import annotations
+
+import math
+from decimal import Decimal, getcontext
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from sapporo2_amd import ics
+
+RNG = np.random.Generator(np.random.PCG64(12345))
+
+
+# ---------------------------------------------------------------------------- exact rounding
+def round_binary(fr: Fraction, mant_bits: int, emin: int) -> float:
+    """Round a rational to the nearest binary floating-point value with `mant_bits`
+    significand bits (ties to even), minimum normal exponent emin."""
+    if fr == 0:
+        return 0.0
+    sign = -1 if fr < 0 else 1
+    a = abs(fr)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    e = max(e, emin)
+    scale = Fraction(2) ** (e - (mant_bits - 1))
+    q = a / scale
+    n = q.numerator // q.denominator
+    rem = q - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return sign * float(n * scale)
+
+
+def f32(fr):
+    return round_binary(Fraction(fr), 24, -126)
+
+
+def f64(fr):
+    return float(Fraction(fr))  # Python rounds rationals to double correctly
+
+
+def fmaf_exact(a, b, c):
+    return f32(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def fma_exact(a, b, c):
+    return f64(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def ds2_mixed_exact(xj, xi):
+    d = [f32(f64(Fraction(xj[k]) - Fraction(xi[k]))) for k in range(3)]
+    return fmaf_exact(d[2], d[2], fmaf_exact(d[1], d[1], f32(Fraction(d[0]) * Fraction(d[0]))))
+
+
+def ds2_f64_exact(xj, xi):
+    d = [f64(Fraction(xj[k]) - Fraction(xi[k])) for k in range(3)]
+    return fma_exact(d[2], d[2], fma_exact(d[1], d[1], f64(Fraction(d[0]) * Fraction(d[0]))))
+
+
+def test_round_binary_matches_numpy_float32():
+    vals = RNG.normal(0, 1e3, 200)
+    for v in vals:
+        assert f32(Fraction(v)) == float(np.float32(v))
+    # products of two float32 are exact in float64, so numpy gives the rounded product
+    a = RNG.normal(0, 1, 200).astype(np.float32)
+    b = RNG.normal(0, 1, 200).astype(np.float32)
+    for x, y in zip(a, b):
+        assert f32(Fraction(float(x)) * Fraction(float(y))) == float(np.float32(x) * np.float32(y))
+
+
+# ---------------------------------------------------------------------------- predictor
+def _taylor_exact(dt, x, v, a, j, s, c, order):
+    dt = Fraction(dt)
+    X, V, A = [], [], []
+    for d in range(3):
+        xs, vs, as_, js = map(Fraction, (x[d], v[d], a[d], j[d]))
+        ss = Fraction(s[d]) if order >= 2 else Fraction(0)
+        cs = Fraction(c[d]) if order >= 2 else Fraction(0)
+        X.append(xs + dt * vs + dt ** 2 / 2 * as_ + dt ** 3 / 6 * js + dt ** 4 / 24 * ss + dt ** 5 / 120 * cs)
+        V.append(vs + dt * as_ + dt ** 2 / 2 * js + dt ** 3 / 6 * ss + dt ** 4 / 24 * cs)
+        A.append(as_ + dt * js + dt ** 2 / 2 * ss + dt ** 3 / 6 * cs)
+    return X, V, A
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_predict_matches_exact_taylor(order):
+    n = 40
+    x, v, a, j, s, c = (RNG.normal(0, 1, (n, 3)) for _ in range(6))
+    tj = RNG.uniform(-0.1, 0.0, n)
+    ti = 0.0
+    px, pv, pa = oracle.predict(order, ti, tj, x, v, a, j, s, c)
+    for k in range(n):
+        X, V, A = _taylor_exact(ti - tj[k], x[k], v[k], a[k], j[k], s[k], c[k], order)
+        for d in range(3):
+            for got, ex, ref in ((px[k, d], X[d], abs(x[k, d]) + abs(v[k, d])),
+                                 (pv[k, d], V[d], abs(v[k, d]) + abs(a[k, d]))):
+                assert abs(Fraction(got) - ex) <= Fraction(8 * 2.0 ** -52) * Fraction(ref), (k, d)
+            if order >= 2:
+                assert abs(Fraction(pa[k, d]) - A[d]) <= Fraction(8 * 2.0 ** -52) * Fraction(
+                    abs(a[k, d]) + abs(j[k, d]))
+
+
+def test_predict_identity_at_dt0_and_exact_dyadic():
+    n = 16
+    x, v, a, j = (RNG.normal(0, 1, (n, 3)) for _ in range(4))
+    t = RNG.uniform(0, 1, n)
+    px, pv, _ = oracle.predict(1, 0.0, np.zeros(n), x, v, a, j)
+    assert np.array_equal(px, x) and np.array_equal(pv, v)
+    # dyadic data, power-of-two dt, zero jerk: every operation is exact
+    xd = np.round(x * 64) / 64
+    vd = np.round(v * 64) / 64
+    ad = np.round(a * 64) / 64
+    dt = 2.0 ** -3
+    px, pv, _ = oracle.predict(1, dt, np.zeros(n), xd, vd, ad, np.zeros((n, 3)))
+    assert np.array_equal(px, xd + dt * vd + dt * dt / 2 * ad)
+    assert np.array_equal(pv, vd + dt * ad)
+    del t
+
+
+def test_predict_h4_ignores_snap_crackle():
+    n = 8
+    x, v, a, j, s, c = (RNG.normal(0, 1, (n, 3)) for _ in range(6))
+    tj = RNG.uniform(-0.1, 0.0, n)
+    p1 = oracle.predict(1, 0.0, tj, x, v, a, j, s, c)
+    p2 = oracle.predict(1, 0.0, tj, x, v, a, j, np.zeros_like(s), np.zeros_like(c))
+    assert np.array_equal(p1[0], p2[0]) and np.array_equal(p1[1], p2[1])
+
+
+# ---------------------------------------------------------------------------- forces
+def _two_body_acc(dr, m, eps2):
+    r2 = np.dot(dr, dr) + eps2
+    return m * dr / r2 ** 1.5
+
+
+def test_two_body_closed_form():
+    m, eps2 = 0.37, 1e-3
+    xi = np.array([[0.1, -0.2, 0.3]])
+    xj = np.array([[0.6, 0.4, -0.5]])
+    vi = np.array([[0.01, 0.02, -0.03]])
+    vj = np.array([[-0.2, 0.1, 0.05]])
+    r = oracle.force(1, xi, vi, [0], xj, vj, [m], [1], eps2, lists=False)
+    dr = xj[0] - xi[0]
+    dv = vj[0] - vi[0]
+    r2 = dr @ dr + eps2
+    np.testing.assert_allclose(r["acc"][0], m * dr / r2 ** 1.5, rtol=1e-15)
+    np.testing.assert_allclose(r["pot"][0], -m / math.sqrt(r2), rtol=1e-15)
+    jerk = m * (dv / r2 ** 1.5 - 3 * (dr @ dv) * dr / r2 ** 2.5)
+    np.testing.assert_allclose(r["jrk"][0], jerk, rtol=1e-14)
+
+
+def _richardson(f, h):
+    """First and second derivatives at 0 by Richardson-extrapolated central differences."""
+    def d1(hh):
+        return (f(hh) - f(-hh)) / (2 * hh)
+
+    def d2(hh):
+        return (f(hh) - 2 * f(0.0) + f(-hh)) / (hh * hh)
+    D1 = (4 * d1(h / 2) - d1(h)) / 3
+    D2 = (4 * d2(h / 2) - d2(h)) / 3
+    return D1, D2
+
+
+@pytest.mark.parametrize("eps2", [0.0, 1e-2])
+def test_jerk_and_snap_are_time_derivatives(eps2):
+    """Along dr(t) = dr0 + dv t + da t^2/2, jerk = d acc/dt and snap = d^2 acc/dt^2."""
+    for trial in range(5):
+        m = RNG.uniform(0.1, 1.0)
+        xi = RNG.normal(0, 0.3, (1, 3)); xj = RNG.normal(0, 0.3, (1, 3)) + np.array([[1.0, 0, 0]])
+        vi = RNG.normal(0, 0.3, (1, 3)); vj = RNG.normal(0, 0.3, (1, 3))
+        ai = RNG.normal(0, 0.3, (1, 3)); aj = RNG.normal(0, 0.3, (1, 3))
+        r = oracle.force(2, xi, vi, [0], xj, vj, [m], [1], eps2, ai=ai, aj=aj, lists=False)
+        dr0, dv, da = xj[0] - xi[0], vj[0] - vi[0], aj[0] - ai[0]
+        D1, D2 = _richardson(lambda t: _two_body_acc(dr0 + dv * t + 0.5 * da * t * t, m, eps2), 1e-3)
+        np.testing.assert_allclose(r["jrk"][0], D1, rtol=1e-7, atol=1e-9 * np.abs(D1).max())
+        np.testing.assert_allclose(r["snp"][0], D2, rtol=1e-5, atol=1e-7 * np.abs(D2).max())
+        # Hermite-4 jerk equals the Hermite-6 jerk (same definition)
+        r4 = oracle.force(1, xi, vi, [0], xj, vj, [m], [1], eps2, lists=False)
+        np.testing.assert_allclose(r4["jrk"][0], r["jrk"][0], rtol=1e-14)
+
+
+def _decimal_forces(xi, vi, ai, idi, xj, vj, aj, mj, idj, eps2):
+    getcontext().prec = 50
+    D = lambda v: Decimal(float(v))  # noqa: E731
+    out = []
+    for i in range(len(xi)):
+        acc = [Decimal(0)] * 3; jrk = [Decimal(0)] * 3; snp = [Decimal(0)] * 3; pot = Decimal(0)
+        for j in range(len(xj)):
+            if idi[i] == idj[j]:
+                continue
+            dr = [D(xj[j][k]) - D(xi[i][k]) for k in range(3)]
+            dv = [D(vj[j][k]) - D(vi[i][k]) for k in range(3)]
+            da = [D(aj[j][k]) - D(ai[i][k]) for k in range(3)]
+            r2 = sum(d * d for d in dr) + D(eps2)
+            r = r2.sqrt()
+            m = D(mj[j])
+            rv = sum(dr[k] * dv[k] for k in range(3))
+            v2 = sum(d * d for d in dv)
+            ra = sum(dr[k] * da[k] for k in range(3))
+            al = rv / r2
+            be = (v2 + ra) / r2 + al * al
+            a_ij = [m * dr[k] / (r2 * r) for k in range(3)]
+            j_ij = [m * dv[k] / (r2 * r) - 3 * al * a_ij[k] for k in range(3)]
+            s_ij = [m * da[k] / (r2 * r) - 6 * al * j_ij[k] - 3 * be * a_ij[k] for k in range(3)]
+            for k in range(3):
+                acc[k] += a_ij[k]; jrk[k] += j_ij[k]; snp[k] += s_ij[k]
+            pot -= m / r
+        out.append(([float(v) for v in acc], [float(v) for v in jrk], [float(v) for v in snp], float(pot)))
+    return out
+
+
+def test_decimal_brute_force_small_n():
+    n = 8
+    x, v, a = (RNG.normal(0, 1, (n, 3)) for _ in range(3))
+    m = RNG.uniform(0.1, 1, n)
+    ids = np.arange(n, dtype=np.int32)
+    eps2 = 1e-3
+    r = oracle.force(2, x, v, ids, x, v, m, ids, eps2, ai=a, aj=a, lists=False)
+    ex = _decimal_forces(x, v, a, ids, x, v, a, m, ids, eps2)
+    for i, (A, J, S, P) in enumerate(ex):
+        np.testing.assert_allclose(r["acc"][i], A, rtol=1e-12, atol=1e-14 * np.abs(A).max())
+        np.testing.assert_allclose(r["jrk"][i], J, rtol=1e-11, atol=1e-13 * np.abs(J).max())
+        np.testing.assert_allclose(r["snp"][i], S, rtol=1e-10, atol=1e-12 * np.abs(S).max())
+        np.testing.assert_allclose(r["pot"][i], P, rtol=1e-13)
+
+
+def test_octahedron_centre():
+    d, m, eps2 = 0.75, 0.2, 1e-4
+    xj = np.array([[d, 0, 0], [-d, 0, 0], [0, d, 0], [0, -d, 0], [0, 0, d], [0, 0, -d]], float)
+    r = oracle.force(1, np.zeros((1, 3)), np.zeros((1, 3)), [99], xj, np.zeros_like(xj),
+                     np.full(6, m), np.arange(6), eps2, lists=False)
+    assert np.array_equal(r["acc"][0], np.zeros(3))
+    np.testing.assert_allclose(r["pot"][0], -6 * m / math.sqrt(d * d + eps2), rtol=1e-15)
+
+
+def test_newton_third_law_and_self_exclusion_by_id():
+    m, x, v = ics.plummer(512, seed=3)
+    a = RNG.normal(0, 1, (512, 3))
+    ids = np.arange(512, dtype=np.int32)
+    r = oracle.force(2, x, v, ids, x, v, m, ids, 1e-4, ai=a, aj=a, lists=False)
+    for k in ("acc", "jrk", "snp"):
+        tot = (m[:, None] * r[k]).sum(0)
+        scale = (m[:, None] * np.abs(r[k])).sum(0)
+        assert np.all(np.abs(tot) <= 1e-13 * scale), k
+    # self-exclusion is by id, not address: distinct ids interact even at the same place
+    r2 = oracle.force(1, x[:1], v[:1], [1000], x[:1], v[:1], m[:1], [7], 1e-4, lists=False)
+    np.testing.assert_allclose(r2["pot"][0], -m[0] / math.sqrt(1e-4), rtol=1e-15)
+    r3 = oracle.force(1, x[:1], v[:1], [7], x[:3], v[:3], m[:3], [7, 7, 8], 1e-4, lists=False)
+    r4 = oracle.force(1, x[:1], v[:1], [7], x[2:3], v[2:3], m[2:3], [8], 1e-4, lists=False)
+    np.testing.assert_array_equal(r3["acc"], r4["acc"])
+
+
+def test_plummer_potential_profile():
+    n = 4096
+    m, x, v = ics.plummer(n, seed=8)
+    ids = np.arange(n, dtype=np.int32)
+    r = oracle.force(1, x, v, ids, x, v, m, ids, 1e-6, lists=False)
+    rad = np.sqrt((x * x).sum(1))
+    for lo, hi in [(0.2, 0.4), (0.5, 0.8), (1.0, 1.5)]:
+        sel = (rad > lo) & (rad < hi)
+        pot_model = ics.plummer_potential(rad[sel]).mean()
+        assert abs(r["pot"][sel].mean() / pot_model - 1) < 0.03
+
+
+# ---------------------------------------------------------------------------- NN and lists
+def _brute_nn_lists(xi, idi, xj, idj, h2, mixed):
+    ni, nj = len(xi), len(xj)
+    nn_a = []; nn_d = []; cnts = []; lists = []
+    for i in range(ni):
+        best = None; ba = -1; cnt = 0; lst = []
+        h = f32(Fraction(h2[i])) if mixed else h2[i]
+        for j in range(nj):
+            if idi[i] == idj[j]:
+                continue
+            d = ds2_mixed_exact(xj[j], xi[i]) if mixed else ds2_f64_exact(xj[j], xi[i])
+            if best is None or d < best:
+                best, ba = d, j
+            if h > d:
+                cnt += 1
+                lst.append(int(idj[j]))
+        nn_a.append(ba); nn_d.append(best); cnts.append(cnt); lists.append(sorted(lst))
+    return nn_a, nn_d, cnts, lists
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_nn_lists_exact_brute_force(mixed):
+    n = 48
+    x = RNG.normal(0, 1, (n, 3)) + 3.0   # offset from the origin: FP64 -> FP32 rounding matters
+    ids = RNG.permutation(n).astype(np.int32)
+    h2 = RNG.uniform(0.1, 2.0, n)
+    if mixed:
+        r = oracle.neighbours_mixed(x, ids, x, ids, h2)
+    else:
+        r = oracle.force(1, x, np.zeros_like(x), ids, x, np.zeros_like(x), np.ones(n), ids, 0.1, h2=h2)
+    nn_a, nn_d, cnts, lists = _brute_nn_lists(x, ids, x, ids, h2, mixed)
+    assert list(r["nn_addr"]) == nn_a
+    assert [float(v) for v in r["nn_ds2"]] == nn_d
+    assert list(r["ngb_cnt"]) == cnts
+    for i in range(n):
+        assert list(r["ngb_list"][i, :cnts[i]]) == lists[i]
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_lattice_ties_and_boundary(mixed):
+    # simple cubic lattice, spacing 1/4 (dyadic): every nearest neighbour is an exact tie
+    g = np.arange(4) / 4.0
+    x = np.array([[a, b, c] for a in g for b in g for c in g]) + 1.5
+    n = len(x)
+    ids = np.arange(n, dtype=np.int32)[::-1].copy()     # ids decreasing with address
+    h2 = np.full(n, 1.0 / 16)                            # exactly the nearest ds2: excluded (strict)
+    fn = (lambda: oracle.neighbours_mixed(x, ids, x, ids, h2)) if mixed else \
+        (lambda: oracle.force(1, x, np.zeros_like(x), ids, x, np.zeros_like(x), np.ones(n), ids, 1.0, h2=h2))
+    r = fn()
+    assert np.all(r["ngb_cnt"] == 0)
+    assert np.all(np.asarray(r["nn_ds2"], float) == 1.0 / 16)
+    # ties go to the smallest address (the first in iteration order)
+    for i in range(n):
+        d = [((x[j] - x[i]) ** 2).sum() if j != i else np.inf for j in range(n)]
+        assert r["nn_addr"][i] == int(np.argmin(d))
+    h2b = np.full(n, np.nextafter(1.0 / 16, 1.0))      # just above: the 6 (or fewer) faces count
+    if mixed:
+        r = oracle.neighbours_mixed(x, ids, x, ids, np.full(n, 1.0 / 16 + 1e-7))
+    else:
+        r = oracle.force(1, x, np.zeros_like(x), ids, x, np.zeros_like(x), np.ones(n), ids, 1.0, h2=h2b)
+    inner = [i for i in range(n) if np.all((x[i] > 1.5) & (x[i] < 2.25))]
+    for i in inner:
+        assert r["ngb_cnt"][i] == 6
+
+
+@pytest.mark.parametrize("k", [255, 256, 257, 300])
+def test_neighbour_overflow_counts(k):
+    """Counts stay exact past the 256 slots; the smallest 256 ids are stored."""
+    xj = np.concatenate([RNG.uniform(-0.1, 0.1, (k, 3)), RNG.uniform(5, 6, (50, 3))])
+    idj = RNG.permutation(len(xj)).astype(np.int32) + 10
+    xi = np.zeros((1, 3))
+    r = oracle.neighbours_mixed(xi, [0], xj, idj, [1.0])
+    assert r["ngb_cnt"][0] == k
+    stored = r["ngb_list"][0][r["ngb_list"][0] >= 0]
+    assert len(stored) == min(k, 256)
+    assert list(stored) == sorted(idj[:k])[:256]
+
+
+def test_api_nn_threshold():
+    nn_addr = np.array([3, -1, 0])
+    nn_ds2 = np.array([0.5, np.inf, 2e10])
+    idj = np.arange(10) + 100
+    nnb, dsmin, found = oracle.api_nn(nn_addr, nn_ds2, idj)
+    assert list(nnb) == [103, 0, 0]
+    assert list(found) == [True, False, False]
+    assert dsmin[0] == 0.5
+
+
+def test_reference_recipe_drand48():
+    """The reference accuracy test's inputs come from glibc drand48 with its default
+    state (test_gravity_block.cpp:40-60); the generator here reproduces glibc's stream."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.seed48.argtypes = [ctypes.POINTER(ctypes.c_ushort)]
+    libc.seed48((ctypes.c_ushort * 3)(0x330E, 0xABCD, 0x1234))   # glibc default state
+    g = ics.Drand48()
+    for _ in range(2000):
+        assert g() == libc.drand48()
+    m, x, v, ids, h2 = ics.drand48_block(64)
+    assert abs(m.sum() - 1.0) < 1e-15 and ids[0] == 1 and np.all(h2 == 0.1)

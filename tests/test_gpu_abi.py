@@ -89,21 +89,28 @@ def test_g6_abi_end_to_end(g6):
         nnb, _, _ = oracle.api_nn(mx["nn_addr"], mx["nn_ds2"], ids)
         assert np.array_equal(inn, nnb)
         # lists through the g6 calls, including the maxlength rule (overflow iff count >= maxlength)
-        assert L.g6_read_neighbour_list_(I(0)) == 0
+        # some outskirts particles have >= 256 neighbours: the read reports the overflow
+        assert L.g6_read_neighbour_list_(I(0)) == int(np.any(mx["ngb_cnt"] >= 256))
         buf = np.zeros(256, np.int32)
         nn = ctypes.c_int(0)
         for k in range(ni):
             ov = L.g6_get_neighbour_list_(I(0), I(k), I(256), ctypes.byref(nn), P(buf, ctypes.c_int))
             c = mx["ngb_cnt"][k]
-            assert ov == 0 and nn.value == c
-            assert np.array_equal(buf[:c], mx["ngb_list"][k, :c])
-        k = int(np.argmax(mx["ngb_cnt"]))
+            assert nn.value == c
+            if c < 256:
+                assert ov == 0 and np.array_equal(buf[:c], mx["ngb_list"][k, :c])
+            else:
+                assert ov == 1
+        small = np.where(mx["ngb_cnt"] < 256, mx["ngb_cnt"], -1)
+        k = int(np.argmax(small))
         c = int(mx["ngb_cnt"][k])
+        assert c > 2
         buf[:] = -7
         ov = L.g6_get_neighbour_list_(I(0), I(k), I(c), ctypes.byref(nn), P(buf, ctypes.c_int))
         assert ov == 1 and nn.value == c and np.all(buf[c:] == -7)
+        buf[:] = -7
         ov = L.g6_get_neighbour_list_(I(0), I(k), I(c - 1), ctypes.byref(nn), P(buf, ctypes.c_int))
-        assert ov == 1 and buf[c - 1] == -7
+        assert ov == 1 and nn.value == c and buf[c - 2] != -7 and buf[c - 1] == -7
         # lasthalf (no NN) gives the same forces and leaves inn untouched
         L.g6calc_firsthalf_(I(0), I(n), I(ni), P(idi, ctypes.c_int), P(xi), P(vi), P(A), P(J), P(pot),
                             D(eps2), P(h2i))

@@ -188,6 +188,118 @@ def cpu_baseline_and_accuracy(w, kept_mix, kept_d, nthreads):
     return cpu_s, acc, out
 
 
+def _timed_calls(eng, fn, reps=2):
+    """Runs fn() reps+1 times; returns the median summed device time (ms) of the last reps."""
+    out = []
+    for r in range(reps + 1):
+        out.append(fn())
+    return float(np.median(out[1:]))
+
+
+def extra_workloads(log):
+    """The other configurations, one measurement each (device-resident Ginteractions/s,
+    mixed and FP64 kernels): Hermite-6 King W0=6 N=262144; a 512-cluster ensemble with
+    one launch per block step; Plummer N=2^20 + 1% central mass on three active sets."""
+    out = {}
+    # Hermite-6, King W0 = 6, N = 262144, all-i (16 calls of 16384)
+    n = 262144
+    m, x, v = ics.king(n, 6.0, seed=81)
+    ids = np.arange(n, dtype=np.int32)
+    z = np.zeros((n, 3))
+    r6 = {}
+    for name, prec in (("mixed", sa.DOUBLESINGLE), ("fp64", sa.DOUBLE)):
+        with sa.Engine(sa.SIXTH, prec) as eng:
+            eng.set_timing(True)
+            eng.set_j_particles(np.arange(n), ids, m, x, v, z, z, z, z)
+            eng.set_time(0.0)
+
+            def sweep():
+                t = 0.0
+                for s0 in range(0, n, sa.NPIPES):
+                    sl = slice(s0, s0 + sa.NPIPES)
+                    eng.start_grav_calc(n, ids[sl], x[sl], v[sl], 1e-6, None, ai=z[sl])
+                    eng.get_grav_results(n, len(ids[sl]), ngb=True)
+                    t += eng.last_device_ms()
+                return t
+            t = _timed_calls(eng, sweep, 1)
+            r6[name] = n * n / (t * 1e-3) / 1e9
+    out["h6_king_w6_n262144"] = {"mixed": round(r6["mixed"], 2), "fp64": round(r6["fp64"], 2),
+                                 "mixed_over_fp64": round(r6["mixed"] / r6["fp64"], 3),
+                                 "flop_per_interaction": 100}
+    log(f"extra h6 king: {out['h6_king_w6_n262144']}")
+    # ensemble: 512 Plummer clusters, N log-uniform in [256, 16384], eps2 = 1e-4
+    sizes = ics.ensemble_sizes(512, 256, 16384, seed=4)
+    cl = [ics.plummer(int(k), seed=1000 + c) for c, k in enumerate(sizes)]
+    rng = np.random.Generator(np.random.PCG64(5))
+    fracs = [2.0 ** -k for k in range(8)]
+    picks = [[np.sort(rng.choice(int(k), max(1, int(round(k * f))), replace=False)) for k in sizes] for f in fracs]
+    re = {}
+    for name, prec in (("mixed", sa.DOUBLESINGLE), ("fp64", sa.DOUBLE)):
+        with sa.Engine(sa.FOURTH, prec) as eng:
+            eng.set_timing(True)
+            offs = eng.set_clusters(sizes)
+            for c, (mc, xc, vc) in enumerate(cl):
+                eng.set_j_particles(offs[c] + np.arange(len(mc)), np.arange(len(mc)), mc, xc, vc)
+            eng.set_time(0.0)
+            eps = np.full(len(sizes), 1e-4)
+            calls = []
+            for pk in picks:
+                cidx = np.concatenate([np.full(len(p), c, np.int32) for c, p in enumerate(pk)])
+                X = np.concatenate([cl[c][1][p] for c, p in enumerate(pk)])
+                V = np.concatenate([cl[c][2][p] for c, p in enumerate(pk)])
+                I = np.concatenate([p for p in pk]).astype(np.int32)
+                inter = sum(len(p) * int(sizes[c]) for c, p in enumerate(pk))
+                calls.append((cidx, I, X, V, inter))
+
+            def sweep():
+                t = 0.0
+                for cidx, I, X, V, inter in calls:
+                    eng.start_grav_calc_clusters(cidx, I, X, V, eps)
+                    eng.get_grav_results(1, len(I), ngb=True)
+                    t += eng.last_device_ms()
+                return t
+            t = _timed_calls(eng, sweep, 1)
+            re[name] = sum(c[4] for c in calls) / (t * 1e-3) / 1e9
+            re["interactions"] = sum(c[4] for c in calls)
+    out["ensemble_512_plummer"] = {"mixed": round(re["mixed"], 2), "fp64": round(re["fp64"], 2),
+                                   "mixed_over_fp64": round(re["mixed"] / re["fp64"], 3),
+                                   "particles": int(sizes.sum()), "launches_per_sweep": len(fracs),
+                                   "interactions_per_sweep": int(re["interactions"])}
+    log(f"extra ensemble: {out['ensemble_512_plummer']}")
+    del cl
+    # Plummer 2^20 + 1% central mass, eps2 = 1e-8, active sets 1024, 32768, all
+    nstar = 1048576
+    m, x, v = ics.plummer_bh(nstar, seed=91, mbh=0.01)
+    n = nstar + 1
+    ids = np.arange(n, dtype=np.int32)
+    h2 = np.concatenate([ics.neighbour_h2(x[:nstar], 32.0, mtot=0.99), [1e-4]])
+    rng = np.random.Generator(np.random.PCG64(92))
+    sets = {"1024": np.sort(rng.choice(n, 1024, replace=False)),
+            "32768": np.sort(rng.choice(n, 32768, replace=False)), "all": np.arange(n)}
+    rb = {}
+    for name, prec in (("mixed", sa.DOUBLESINGLE), ("fp64", sa.DOUBLE)):
+        with sa.Engine(sa.FOURTH, prec) as eng:
+            eng.set_timing(True)
+            eng.set_j_particles(np.arange(n), ids, m, x, v)
+            eng.set_time(0.0)
+            for key, sel in sets.items():
+                def sweep():
+                    t = 0.0
+                    for s0 in range(0, len(sel), sa.NPIPES):
+                        ch = sel[s0:s0 + sa.NPIPES]
+                        eng.start_grav_calc(n, ids[ch], x[ch], v[ch], 1e-8, h2[ch])
+                        eng.get_grav_results(n, len(ch), ngb=True)
+                        t += eng.last_device_ms()
+                    return t
+                t = _timed_calls(eng, sweep, 1 if key == "all" else 2)
+                rb.setdefault(key, {})[name] = round(len(sel) * n / (t * 1e-3) / 1e9, 2)
+    for key in rb:
+        rb[key]["mixed_over_fp64"] = round(rb[key]["mixed"] / rb[key]["fp64"], 3)
+    out["plummer_2e20_central_mass"] = rb
+    log(f"extra plummer+BH: {rb}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +307,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / accuracy leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the other configurations")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="run one kernel only (profiling)")
     args = ap.parse_args()
@@ -288,6 +401,8 @@ def main():
         line["checks"] = checks
     if full_cmp:
         line["mixed_vs_fp64_acc_full_N"] = full_cmp
+    if not args.no_extras and not args.only:
+        line["extra_workloads"] = extra_workloads(log)
     print(json.dumps(line), flush=True)
 
 

@@ -59,6 +59,20 @@ int sapamd_start_grav_calc(sapamd_engine *e, int nj, int ni, const int *id, cons
 int sapamd_get_grav_results(sapamd_engine *e, int nj, int ni, double *acc, double *jrk,
                             double *snp, double *crk, double *pot, int *nnb, double *dsmin,
                             int ngb);
+/* Ensembles of independent clusters on one engine (no reference counterpart:
+ * the reference ignores cluster_id, sapporoG6lib.cpp:99, and holds one system
+ * per process).  set_clusters lays out n clusters of nj[c] j-particles;
+ * cluster c owns the addresses [cluster_offset(c), cluster_offset(c) + nj[c])
+ * of set_j_particle.  start_grav_calc_clusters evaluates ni i-particles
+ * grouped by cluster (cluster[] non-decreasing), each against its own
+ * cluster only with softening eps2[cluster]; results, nearest neighbours and
+ * lists are read with sapamd_get_grav_results / sapamd_get_ngb_list.
+ * ni <= 4194304 per call. */
+int sapamd_set_clusters(sapamd_engine *e, int n, const int *nj);
+int sapamd_cluster_offset(sapamd_engine *e, int c);
+int sapamd_start_grav_calc_clusters(sapamd_engine *e, int ni, const int *cluster, const int *id,
+                                    const double *xi, const double *vi, const double *ai,
+                                    const double *eps2, const double *h2);
 /* sapporohostclass.h:170-171  returns 1 on overflow (a count >= 256). */
 int sapamd_read_ngb_list(sapamd_engine *e);
 /* sapporohostclass.h:172-176  returns 1 when the count >= maxlen. */
@@ -76,6 +90,9 @@ int sapamd_retrieve_j_particle_state(sapamd_engine *e, int addr, double *mass, d
 /* Launch shape and instrumentation (no reference counterpart; the reference
  * fixes its shape in sapporohostclass.cpp:1499-1561). */
 int sapamd_set_force_override(sapamd_engine *e, int P, int S);
+/* on: the 4 waves of a work-group share one j-range; off (default, faster on
+ * MI355X): they take consecutive j-splits of one i-group */
+int sapamd_set_split_major(sapamd_engine *e, int on);
 int sapamd_set_timing(sapamd_engine *e, int on);
 float sapamd_last_force_ms(sapamd_engine *e);
 float sapamd_last_predict_ms(sapamd_engine *e);
@@ -83,6 +100,11 @@ float sapamd_last_predict_ms(sapamd_engine *e);
  * copies to the end of the finalize kernel (copies excluded) */
 float sapamd_last_device_ms(sapamd_engine *e);
 int sapamd_last_config(sapamd_engine *e, int *P, int *S, int *jchunk);
+/* Launch-shape table (sapporo2_amd/autotune.py output; loaded at open from
+ * $SAPPORO2_AMD_TUNE or tune/<arch>.txt beside the library).  Returns the
+ * number of entries read (0: heuristic shapes). */
+int sapamd_load_tuning(sapamd_engine *e, const char *path);
+int sapamd_tuning_entries(sapamd_engine *e);
 int sapamd_num_cus(sapamd_engine *e);
 int sapamd_synchronize(sapamd_engine *e);
 const char *sapamd_last_error(sapamd_engine *e);

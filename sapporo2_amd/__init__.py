@@ -51,9 +51,10 @@ ENGINE_SYMBOLS = [
     "sapamd_set_time", "sapamd_set_no_time", "sapamd_set_j_particle", "sapamd_start_grav_calc",
     "sapamd_get_grav_results", "sapamd_set_j_particles", "sapamd_last_device_ms", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
-    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_timing",
-    "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_num_cus",
-    "sapamd_synchronize", "sapamd_last_error",
+    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_timing",
+    "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
+    "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
+    "sapamd_start_grav_calc_clusters",
 ]
 
 
@@ -83,6 +84,9 @@ def lib():
                                              ctypes.c_double, _dp]
         L.sapamd_get_grav_results.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                               _dp, _ip, _dp, ctypes.c_int]
+        L.sapamd_set_clusters.argtypes = [_vp, ctypes.c_int, _ip]
+        L.sapamd_cluster_offset.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_start_grav_calc_clusters.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
         L.sapamd_read_ngb_list.argtypes = [_vp]
         L.sapamd_get_ngb_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip]
         L.sapamd_force_prediction.argtypes = [_vp, ctypes.c_int]
@@ -91,12 +95,15 @@ def lib():
         L.sapamd_retrieve_j_particle_state.argtypes = [_vp, ctypes.c_int] + [_dp] * 10
         L.sapamd_set_force_override.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.sapamd_set_timing.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_set_split_major.argtypes = [_vp, ctypes.c_int]
         L.sapamd_last_force_ms.argtypes = [_vp]
         L.sapamd_last_force_ms.restype = ctypes.c_float
         L.sapamd_last_predict_ms.argtypes = [_vp]
         L.sapamd_last_predict_ms.restype = ctypes.c_float
         L.sapamd_last_config.argtypes = [_vp, _ip, _ip, _ip]
         L.sapamd_num_cus.argtypes = [_vp]
+        L.sapamd_load_tuning.argtypes = [_vp, ctypes.c_char_p]
+        L.sapamd_tuning_entries.argtypes = [_vp]
         L.sapamd_synchronize.argtypes = [_vp]
         L.sapamd_last_error.argtypes = [_vp]
         L.sapamd_last_error.restype = ctypes.c_char_p
@@ -237,6 +244,22 @@ class Engine:
                     out[k][s:e] = v
         return out
 
+    # --- ensembles of independent clusters ----------------------------------------------
+    def set_clusters(self, sizes):
+        sizes = np.ascontiguousarray(sizes, np.int32)
+        self._check(self._L.sapamd_set_clusters(self._h, len(sizes), _pi(sizes)), "set_clusters")
+        return np.array([self._L.sapamd_cluster_offset(self._h, c) for c in range(len(sizes))])
+
+    def start_grav_calc_clusters(self, cluster, ids, xi, vi, eps2, h2=None, ai=None):
+        ni = len(ids)
+        self._keep = (np.ascontiguousarray(cluster, np.int32), np.ascontiguousarray(ids, np.int32),
+                      _d(xi, (ni, 3)), _d(vi, (ni, 3)), None if ai is None else _d(ai, (ni, 3)),
+                      np.ascontiguousarray(eps2, np.float64), None if h2 is None else _d(h2, (ni,)))
+        cl, ids_, xi_, vi_, ai_, e2, h2_ = self._keep
+        return self._check(self._L.sapamd_start_grav_calc_clusters(
+            self._h, ni, _pi(cl), _pi(ids_), _pd(xi_), _pd(vi_), _pd(ai_), _pd(e2), _pd(h2_)),
+            "start_grav_calc_clusters")
+
     def read_ngb_list(self):
         return self._check(self._L.sapamd_read_ngb_list(self._h), "read_ngb_list")
 
@@ -273,6 +296,9 @@ class Engine:
     def set_force_override(self, P=0, S=0):
         self._L.sapamd_set_force_override(self._h, int(P), int(S))
 
+    def set_split_major(self, on=True):
+        self._L.sapamd_set_split_major(self._h, 1 if on else 0)
+
     def set_timing(self, on=True):
         self._L.sapamd_set_timing(self._h, 1 if on else 0)
 
@@ -289,6 +315,14 @@ class Engine:
         P = ctypes.c_int(); S = ctypes.c_int(); jc = ctypes.c_int()
         self._L.sapamd_last_config(self._h, ctypes.byref(P), ctypes.byref(S), ctypes.byref(jc))
         return P.value, S.value, jc.value
+
+    def load_tuning(self, path):
+        """Load a launch-shape table (None or a missing file: heuristic shapes)."""
+        return self._L.sapamd_load_tuning(self._h, None if path is None else str(path).encode())
+
+    @property
+    def tuning_entries(self):
+        return self._L.sapamd_tuning_entries(self._h)
 
     @property
     def num_cus(self):

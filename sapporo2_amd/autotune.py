@@ -1,0 +1,132 @@
+"""Launch-shape autotuner for the force path.
+
+For each integration order, working precision, neighbour-list mode and active-count
+bucket ni = 2^k (k = 0 .. 14), it times candidate lane shapes P (i-particles per wave)
+and j-splits S on a Plummer system of N particles, and keeps the shape with the
+smallest device time of the whole call (scatter/reset, force, finalize: the engine's
+device window).  The result is a text table the engine reads at open():
+
+    # order precision ngb log2(ni) log2(nj) P log2(j per split)
+    1 1 1 14 17 64 10
+
+usage (on the GPU): python -m sapporo2_amd.autotune [--n 131072] [--out PATH]
+The default output is tune/<arch>.txt beside the library, the file the engine loads.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+import sapporo2_amd as sa
+from sapporo2_amd import ics
+
+
+def candidates(ni, nj):
+    pnat = 1
+    while pnat < min(ni, 64):
+        pnat <<= 1
+    out = []
+    for P in sorted({pnat, max(1, pnat // 2), max(1, pnat // 4)}):
+        Q = 64 // P
+        G = (ni + P - 1) // P
+        for k in range(0, 16):
+            S = 1 << k
+            jchunk = math.ceil(nj / S)
+            if jchunk < max(64, 16 * Q):
+                break
+            waves = G * S
+            if waves < 128 or waves > 262144:
+                continue
+            out.append((P, S))
+    return out
+
+
+def time_call(eng, nj, ids, x, v, h2, sel, eps2, reps):
+    ts = []
+    for r in range(reps + 1):
+        eng.start_grav_calc(nj, ids[sel], x[sel], v[sel], eps2, None if h2 is None else h2[sel],
+                            ai=None if eng.order < sa.SIXTH else np.zeros((len(sel), 3)))
+        eng.get_grav_results(nj, len(sel), ngb=True)
+        if r:
+            ts.append(eng.last_device_ms())
+    return float(np.median(ts))
+
+
+def tune(n=131072, orders=(sa.FOURTH, sa.SIXTH), precs=(sa.DOUBLESINGLE, sa.DOUBLE), reps=3,
+         kmax=14, seed=1, log=print):
+    m, x, v = ics.plummer(n, seed=seed)
+    ids = np.arange(n, dtype=np.int32)
+    h2 = ics.neighbour_h2(x, 32.0)
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    rows, table = [], []
+    lj = int(math.ceil(math.log2(n)))
+    for order in orders:
+        for prec in precs:
+            eng = sa.Engine(order, prec)
+            eng.load_tuning(None)
+            eng.set_timing(True)
+            eng.set_j_particles(np.arange(n), ids, m, x, v, np.zeros((n, 3)), np.zeros((n, 3)),
+                                np.zeros((n, 3)) if order >= sa.SIXTH else None,
+                                np.zeros((n, 3)) if order >= sa.SIXTH else None)
+            eng.set_time(0.0)
+            for ngb in (1, 0):
+                for k in range(kmax + 1):
+                    ni = 1 << k
+                    sel = np.sort(rng.choice(n, ni, replace=False))
+                    best = None
+                    for P, S in candidates(ni, n):
+                        eng.set_force_override(P, S)
+                        t = time_call(eng, n, ids, x, v, h2 if ngb else None, sel, 1e-6, reps)
+                        _, S_used, jc = eng.last_config()
+                        rows.append(dict(order=order, prec=prec, ngb=ngb, ni=ni, P=P, S=S_used,
+                                         jchunk=jc, dev_ms=t))
+                        if best is None or t < best[0]:
+                            best = (t, P, jc)
+                    eng.set_force_override(0, 0)
+                    t, P, jc = best
+                    jlog = int(round(math.log2(jc)))
+                    table.append((order, prec, ngb, k, lj, P, jlog))
+                    log(f"order {order} prec {prec} ngb {ngb} ni {ni}: P {P} jchunk {jc} "
+                        f"{t * 1e3:.1f} us ({ni * n / t / 1e6:.0f} Ginter/s)")
+            eng.close()
+    return table, rows
+
+
+def write_table(path, table, n):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(f"# launch shapes tuned by sapporo2_amd/autotune.py on a Plummer N={n} system\n")
+        f.write("# order precision ngb log2(ni) log2(nj) P log2(j per split)\n")
+        for t in table:
+            f.write(" ".join(str(int(v)) for v in t) + "\n")
+
+
+def main():
+    ap = argparse.ArgumentParser(description="force launch-shape autotuner")
+    ap.add_argument("--n", type=int, default=131072)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--json", default=None, help="also write every measurement here")
+    ap.add_argument("--orders", default="1,2")
+    ap.add_argument("--precs", default="1,2")
+    ap.add_argument("--reps", type=int, default=3)
+    args = ap.parse_args()
+    t0 = time.time()
+    table, rows = tune(args.n, tuple(int(o) for o in args.orders.split(",")),
+                       tuple(int(p) for p in args.precs.split(",")), args.reps,
+                       log=lambda s: print(s, flush=True))
+    out = args.out or os.path.join(sa.HERE, "tune", "gfx950.txt")
+    write_table(out, table, args.n)
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(rows, f)
+    print(f"wrote {out} ({len(table)} entries) in {time.time() - t0:.1f} s", flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

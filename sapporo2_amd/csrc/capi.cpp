@@ -53,6 +53,15 @@ int sapamd_get_grav_results(sapamd_engine* e, int nj, int ni, double* acc, doubl
            : -1;
 }
 
+int sapamd_set_clusters(sapamd_engine* e, int n, const int* nj) { return e ? e->eng.set_clusters(n, nj) : -1; }
+int sapamd_cluster_offset(sapamd_engine* e, int c) { return e ? e->eng.cluster_offset(c) : -1; }
+int sapamd_start_grav_calc_clusters(sapamd_engine* e, int ni, const int* cluster, const int* id, const double* xi,
+                                    const double* vi, const double* ai, const double* eps2, const double* h2) {
+  return e ? e->eng.start_grav_calc_clusters(ni, cluster, id, reinterpret_cast<V3>(xi), reinterpret_cast<V3>(vi),
+                                             reinterpret_cast<V3>(ai), eps2, h2)
+           : -1;
+}
+
 int sapamd_read_ngb_list(sapamd_engine* e) { return e ? e->eng.read_ngb_list() : -1; }
 int sapamd_get_ngb_list(sapamd_engine* e, int ipipe, int maxlen, int* n, int* list) {
   return e ? e->eng.get_ngb_list(ipipe, maxlen, n, list) : -1;
@@ -76,6 +85,11 @@ int sapamd_set_force_override(sapamd_engine* e, int P, int S) {
   e->eng.set_force_override(P, S);
   return 0;
 }
+int sapamd_set_split_major(sapamd_engine* e, int on) {
+  if (!e) return -1;
+  e->eng.set_split_major(on != 0);
+  return 0;
+}
 int sapamd_set_timing(sapamd_engine* e, int on) {
   if (!e) return -1;
   e->eng.set_timing(on != 0);
@@ -92,6 +106,8 @@ int sapamd_last_config(sapamd_engine* e, int* P, int* S, int* jchunk) {
   if (jchunk) *jchunk = c.jchunk;
   return 0;
 }
+int sapamd_load_tuning(sapamd_engine* e, const char* path) { return e ? e->eng.load_tuning(path) : -1; }
+int sapamd_tuning_entries(sapamd_engine* e) { return e ? e->eng.tuning_entries() : 0; }
 int sapamd_num_cus(sapamd_engine* e) { return e ? e->eng.num_cus() : 0; }
 int sapamd_synchronize(sapamd_engine* e) { return e ? e->eng.synchronize() : -1; }
 const char* sapamd_last_error(sapamd_engine* e) { return e ? e->eng.last_error() : "null handle"; }

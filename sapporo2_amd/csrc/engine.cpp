@@ -1,6 +1,8 @@
 // engine.cpp -- host engine (see engine.h).  Reference behaviour cited per method.
 #include "engine.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -18,8 +20,9 @@ using namespace sapk;
   } while (0)
 
 namespace {
-constexpr int WAVES_PER_SIMD_TARGET = 8;
+constexpr int WAVES_PER_SIMD_TARGET = 32;   // several dispatch rounds balance better
 constexpr int MIN_ITERS_PER_LANE = 16;
+constexpr int MIN_J_PER_SPLIT = 512;
 constexpr int FOLD = 64;   // interactions per lane between FP64 folds (kernels.hip KT)
 constexpr int IBUF_FIELDS = 10;  // xi 3, vi 3, ai 3, h2 1
 
@@ -79,6 +82,22 @@ int Engine::open(int device, int order, int precision) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
   cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  arch_ = prop.gcnArchName;
+  arch_ = arch_.substr(0, arch_.find(':'));
+  {
+    // tuned launch shapes: $SAPPORO2_AMD_TUNE, else tune/<arch>.txt beside the library
+    std::string path;
+    if (const char* env = getenv("SAPPORO2_AMD_TUNE")) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (dladdr(reinterpret_cast<void*>(&sapk::launch_force), &info) && info.dli_fname) {
+        path = info.dli_fname;
+        path = path.substr(0, path.rfind('/') + 1) + "tune/" + arch_ + ".txt";
+      }
+    }
+    if (!path.empty()) load_tuning(path.c_str());
+  }
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   HIPCHK(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(hipEventCreate(&ev_d0_), "hipEventCreate");
@@ -89,20 +108,8 @@ int Engine::open(int device, int order, int precision) {
   HIPCHK(hipEventCreate(&ev_p1_), "hipEventCreate");
   for (auto& s : stage_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
 
-  const size_t nI = NPIPES;
-  HIPCHK(hmalloc(&ibuf_h_, IBUF_FIELDS * nI), "pinned i buffer");
-  HIPCHK(hmalloc(&iid_h_, nI), "pinned i ids");
-  HIPCHK(dmalloc(&ibuf_d_, IBUF_FIELDS * nI), "i buffer");
-  HIPCHK(dmalloc(&iid_d_, nI), "i ids");
-  HIPCHK(dmalloc(&out_d_, nI * out_fields(order_)), "result buffer");
-  HIPCHK(hmalloc(&out_h_, nI * out_fields(order_)), "pinned results");
-  HIPCHK(dmalloc(&cnt_d_, nI), "neighbour counts");
-  HIPCHK(dmalloc(&list_d_, nI * NGB_CAP), "neighbour lists");
-  HIPCHK(dmalloc(&off_d_, nI + 1), "neighbour offsets");
-  HIPCHK(dmalloc(&packed_d_, nI * NGB_CAP), "packed lists");
-  HIPCHK(hmalloc(&off_h_, nI + 1), "pinned offsets");
-  HIPCHK(hmalloc(&packed_h_, nI * NGB_CAP), "pinned packed lists");
-  HIPCHK(hipMemsetAsync(cnt_d_, 0, nI * sizeof(int), stream_), "memset");
+  icap_ = 0;
+  if (ensure_icap(NPIPES) != 0) return -1;
 
   nj_max_ = NJ_INITIAL;
   max_addr_ = -1;
@@ -123,7 +130,7 @@ int Engine::close() {
   if (!open_) return 0;
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
-  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc);
+  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
   st_.cap = 0;
   for (auto& s : stage_) {
     hfree(s.d); hfree(s.ai); s.cap = 0; s.busy = false;
@@ -137,6 +144,8 @@ int Engine::close() {
   dfree(cnt_d_); dfree(list_d_); dfree(off_d_); dfree(packed_d_);
   hfree(off_h_); hfree(packed_h_);
   dfree(gather_d_); gather_cap_ = 0;
+  dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
+  icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
   for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
@@ -161,7 +170,7 @@ int Engine::ensure_store(int need) {
   if (need <= st_.cap) return 0;
   const int oldcap = st_.cap;
   long nc = std::max<long>({(long)need, (long)oldcap + oldcap / 2, (long)NJ_INITIAL});
-  nc = (nc + 255) / 256 * 256;
+  nc = (nc + 255) / 256 * 256;  // a multiple of JTILE and of the predictor's block
   const int newcap = (int)nc;
   const int nfr = raw_fields(order_);
   const size_t rb = jrec_bytes(order_, f64_);
@@ -170,6 +179,9 @@ int Engine::ensure_store(int need) {
   HIPCHK(dmalloc(&ns.raw, (size_t)nfr * newcap), "j-store raw");
   HIPCHK(dmalloc(&ns.id, (size_t)newcap), "j-store ids");
   HIPCHK(hipMalloc(&ns.rec, rb * newcap), "j-store records");
+  HIPCHK(dmalloc(&ns.idr, (size_t)newcap / JTILE), "j-store tile id ranges");
+  // zeroed slots carry id 0, so every new tile's range starts at [0, 0]
+  HIPCHK(hipMemsetAsync(ns.idr, 0, sizeof(int2) * (size_t)newcap / JTILE, stream_), "memset");
   if (!f64_) {
     HIPCHK(dmalloc(&ns.pvel, (size_t)3 * newcap), "j-store pvel");
     if (order_ >= ORDER_H6) HIPCHK(dmalloc(&ns.pacc, (size_t)3 * newcap), "j-store pacc");
@@ -185,6 +197,8 @@ int Engine::ensure_store(int need) {
                             sizeof(double) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
     HIPCHK(hipMemcpyAsync(ns.id, st_.id, sizeof(int) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
     HIPCHK(hipMemcpyAsync(ns.rec, st_.rec, rb * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
+    HIPCHK(hipMemcpyAsync(ns.idr, st_.idr, sizeof(int2) * (size_t)oldcap / JTILE, hipMemcpyDeviceToDevice,
+                          stream_), "store copy");
     for (int d = 0; d < 3; ++d) {
       if (ns.pvel)
         HIPCHK(hipMemcpyAsync(ns.pvel + (size_t)d * newcap, st_.pvel + (size_t)d * oldcap,
@@ -195,7 +209,7 @@ int Engine::ensure_store(int need) {
     }
     HIPCHK(hipStreamSynchronize(stream_), "store grow sync");
   }
-  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc);
+  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
   st_ = ns;
   return 0;
 }
@@ -387,19 +401,67 @@ int Engine::ensure_partials(size_t entries) {
 // Launch shape: P i-particles per wave (64 unless fewer are active), S j-splits
 // so that about WAVES_PER_SIMD_TARGET waves per SIMD exist, each split at least
 // MIN_ITERS_PER_LANE iterations per lane.
-ForceConfig Engine::choose_config(int ni, int nj) const {
+int Engine::load_tuning(const char* path) {
+  tune_.clear();
+  tune_path_.clear();
+  FILE* f = path ? fopen(path, "r") : nullptr;
+  if (!f) return 0;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    TuneEntry t{};
+    if (sscanf(line, "%d %d %d %d %d %d %d", &t.order, &t.prec, &t.ngb, &t.li, &t.lj, &t.P, &t.jlog) == 7 &&
+        force_p_supported(t.P) && t.jlog >= 0 && t.jlog < 31)
+      tune_.push_back(t);
+  }
+  fclose(f);
+  tune_path_ = path;
+  return (int)tune_.size();
+}
+
+static int ceil_log2(long v) {
+  int l = 0;
+  while ((1L << l) < v) ++l;
+  return l;
+}
+
+// Launch shape: the tuned table's entry for (order, precision, neighbour lists,
+// active-count bucket, nearest nj) when present; otherwise P i-particles per
+// wave (64 unless fewer are active) and S j-splits for ~WAVES_PER_SIMD_TARGET
+// waves per SIMD with at least MIN_J_PER_SPLIT j per split.
+ForceConfig Engine::choose_config(int ni, int nj, bool ngb) const {
   int P = 64;
   if (ni < 64) {
     P = 1;
     while (P < ni) P <<= 1;
   }
+  long jchunk_t = 0;
+  if (!tune_.empty() && ovr_P_ <= 0 && ovr_S_ <= 0) {
+    const int li = ceil_log2(ni), lj = ceil_log2(nj);
+    const int prec = f64_ ? PREC_DOUBLE : PREC_DS;
+    const TuneEntry* best = nullptr;
+    for (const TuneEntry& t : tune_)
+      if (t.order == order_ && t.prec == prec && t.ngb == (ngb ? 1 : 0) && t.li == li &&
+          (!best || std::abs(t.lj - lj) < std::abs(best->lj - lj)))
+        best = &t;
+    if (best) {
+      P = best->P;
+      jchunk_t = 1L << best->jlog;
+    }
+  }
   if (ovr_P_ > 0 && force_p_supported(ovr_P_)) P = ovr_P_;
   const int Q = 64 / P;
   const long G = (ni + P - 1) / P;
-  const long target = (long)cus_ * 4 * WAVES_PER_SIMD_TARGET;
-  long S = (target + G - 1) / G;
-  const long maxS = std::max<long>(1, (nj + (long)Q * MIN_ITERS_PER_LANE - 1) / ((long)Q * MIN_ITERS_PER_LANE));
-  S = std::min(S, maxS);
+  long S;
+  if (jchunk_t > 0) {
+    S = (nj + jchunk_t - 1) / jchunk_t;
+  } else {
+    const long target = (long)cus_ * 4 * WAVES_PER_SIMD_TARGET;
+    S = (target + G - 1) / G;
+    const long minj = std::max<long>((long)Q * MIN_ITERS_PER_LANE, MIN_J_PER_SPLIT);
+    const long maxS = std::max<long>(1, (nj + minj - 1) / minj);
+    S = std::min(S, maxS);
+  }
   if (ovr_S_ > 0) S = ovr_S_;
   if (S < 1) S = 1;
   long jchunk = (nj + S - 1) / S;
@@ -409,7 +471,211 @@ ForceConfig Engine::choose_config(int ni, int nj) const {
   if (jchunk < 1) jchunk = 1;
   S = (nj + jchunk - 1) / jchunk;
   if (S < 1) S = 1;
-  return ForceConfig{P, (int)S, (int)jchunk};
+  return ForceConfig{P, (int)S, (int)jchunk, ovr_smajor_};
+}
+
+// i-particle-sized buffers (inputs, result rows, neighbour lists) for up to
+// `need` i-particles per call.
+int Engine::ensure_icap(int need) {
+  if (need <= icap_) return 0;
+  if (stream_) HIPCHK(hipStreamSynchronize(stream_), "i-buffer grow sync");
+  inflight_ = false;
+  hfree(ibuf_h_); hfree(iid_h_); dfree(ibuf_d_); dfree(iid_d_); dfree(out_d_); hfree(out_h_);
+  dfree(cnt_d_); dfree(list_d_); dfree(off_d_); dfree(packed_d_); hfree(off_h_); hfree(packed_h_);
+  dfree(idesc_d_);
+  const size_t nI = (size_t)need;
+  HIPCHK(hmalloc(&ibuf_h_, IBUF_FIELDS * nI), "pinned i buffer");
+  HIPCHK(hmalloc(&iid_h_, nI), "pinned i ids");
+  HIPCHK(dmalloc(&ibuf_d_, IBUF_FIELDS * nI), "i buffer");
+  HIPCHK(dmalloc(&iid_d_, nI), "i ids");
+  HIPCHK(dmalloc(&out_d_, nI * out_fields(order_)), "result buffer");
+  HIPCHK(hmalloc(&out_h_, nI * out_fields(order_)), "pinned results");
+  HIPCHK(dmalloc(&cnt_d_, nI), "neighbour counts");
+  HIPCHK(dmalloc(&list_d_, nI * NGB_CAP), "neighbour lists");
+  HIPCHK(dmalloc(&off_d_, nI + 1), "neighbour offsets");
+  HIPCHK(dmalloc(&packed_d_, nI * NGB_CAP), "packed lists");
+  HIPCHK(hmalloc(&off_h_, nI + 1), "pinned offsets");
+  HIPCHK(hmalloc(&packed_h_, nI * NGB_CAP), "pinned packed lists");
+  HIPCHK(dmalloc(&idesc_d_, nI), "ensemble i descriptors");
+  HIPCHK(hipMemsetAsync(cnt_d_, 0, nI * sizeof(int), stream_), "memset");
+  icap_ = need;
+  have_results_ = false;
+  return 0;
+}
+
+// Ensemble layout: cluster c owns the addresses [offset(c), offset(c) + nj_c),
+// offsets aligned to the JTILE tile.  The reference has no counterpart (its
+// GRAPE-6 ABI ignores cluster_id, sapporoG6lib.cpp:99).
+int Engine::set_clusters(int n, const int* njc) {
+  if (!open_) return fail("set_clusters: library not open");
+  if (n <= 0 || !njc) return fail("set_clusters: bad arguments");
+  cl_off_.assign(n, 0);
+  cl_n_.assign(n, 0);
+  long off = 0;
+  for (int c = 0; c < n; ++c) {
+    if (njc[c] < 0) return fail("set_clusters: negative cluster size");
+    cl_off_[c] = (int)off;
+    cl_n_[c] = njc[c];
+    off += (njc[c] + JTILE - 1) / JTILE * JTILE;
+    if (off > (1L << 30)) return fail("set_clusters: too many particles");
+  }
+  nj_max_ = std::max(nj_max_, (int)off);
+  if ((int)slot_of_.size() < nj_max_) slot_of_.resize(nj_max_, -1);
+  if (ensure_store((int)off) != 0) return -1;
+  cl_total_ = (int)off;
+  return 0;
+}
+
+int Engine::cluster_offset(int c) const {
+  return (c >= 0 && c < (int)cl_off_.size()) ? cl_off_[c] : -1;
+}
+
+// Forces for i-particles of many clusters in one launch.  The i-particles
+// are grouped by cluster (non-decreasing cluster index); each interacts with
+// its own cluster's j-particles only, with that cluster's eps2.
+int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, const double (*xi)[3],
+                                     const double (*vi)[3], const double (*ai)[3], const double* eps2c,
+                                     const double* h2) {
+  if (!open_) return fail("start_grav_calc_clusters: library not open");
+  if (cl_off_.empty()) return fail("start_grav_calc_clusters: no cluster layout (set_clusters)");
+  if (ni == 0) return 0;
+  if (ni < 0 || ni > ENS_MAX_I || !cluster || !eps2c) return fail("start_grav_calc_clusters: bad arguments");
+  const int nc = (int)cl_off_.size();
+  for (int i = 0; i < ni; ++i) {
+    if (cluster[i] < 0 || cluster[i] >= nc) return fail("start_grav_calc_clusters: cluster index out of range");
+    if (i && cluster[i] < cluster[i - 1]) return fail("start_grav_calc_clusters: i-particles not grouped by cluster");
+  }
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (inflight_ && wait_results() != 0) return -1;
+  if (ensure_icap(ni) != 0) return -1;
+
+  double* bx = ibuf_h_;
+  double* bv = bx + 3 * (size_t)ni;
+  double* ba = bv + 3 * (size_t)ni;
+  double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
+  bool ngb_on = false;
+  for (int i = 0; i < ni; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      bx[3 * i + d] = xi[i][d];
+      bv[3 * i + d] = vi ? vi[i][d] : 0.0;
+      if (order_ >= ORDER_H6) ba[3 * i + d] = ai ? ai[i][d] : 0.0;
+    }
+    const double hh = h2 ? h2[i] : 0.0;
+    bh[i] = hh;
+    ngb_on |= hh > 0.0;
+    iid_h_[i] = id ? id[i] : -1;
+  }
+  const size_t nd = (size_t)(bh + ni - bx);
+  HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
+  HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
+
+  // work table: per active cluster, i-groups of P (by its active count) x j-splits
+  struct Grp { int c, ib, n, P, G; };
+  std::vector<Grp> grps;
+  long work = 0;
+  for (int i = 0; i < ni;) {
+    int e = i;
+    while (e < ni && cluster[e] == cluster[i]) ++e;
+    const int n = e - i;
+    const int P = n > 32 ? 64 : n > 8 ? 16 : n > 2 ? 4 : 1;
+    const int G = (n + P - 1) / P;
+    grps.push_back(Grp{cluster[i], i, n, P, G});
+    work += (long)G * cl_n_[cluster[i]];
+    i = e;
+  }
+  const long target = (long)cus_ * 4 * WAVES_PER_SIMD_TARGET;
+  long jch = std::max<long>(MIN_J_PER_SPLIT, (work + target - 1) / target);
+  jch = (jch + JTILE * 16 - 1) / (JTILE * 16) * (JTILE * 16);  // whole 64-interaction folds for Q <= 16
+  ens_items_.clear();
+  ens_idesc_.assign(ni, int4{0, 0, 0, 0});
+  for (const Grp& g : grps) {
+    const int off = cl_off_[g.c], nj = cl_n_[g.c];
+    const int S = std::max(1, (int)((nj + jch - 1) / jch));
+    for (int q = 0; q < g.G; ++q) {
+      const int first = (int)ens_items_.size();
+      const int i0 = g.ib + q * g.P;
+      const int cnt = std::min(g.P, g.n - q * g.P);
+      for (int s = 0; s < S; ++s) {
+        const int jb = off + (int)std::min<long>((long)s * jch, nj);
+        const int je = off + (int)std::min<long>((long)(s + 1) * jch, nj);
+        ens_items_.push_back(EnsItem{i0, cnt, jb, je, g.P, g.c, 0, 0});
+      }
+      for (int k = 0; k < cnt; ++k) ens_idesc_[i0 + k] = int4{first, S, k, 0};
+    }
+  }
+  const int nitems = (int)ens_items_.size();
+  if (ens_items_cap_ < nitems) {
+    dfree(ens_items_d_);
+    ens_items_cap_ = std::max(nitems, 2 * ens_items_cap_);
+    HIPCHK(dmalloc(&ens_items_d_, (size_t)ens_items_cap_), "ensemble work table");
+  }
+  if (ens_eps_cap_ < nc) {
+    dfree(ens_eps_d_);
+    ens_eps_cap_ = nc;
+    HIPCHK(dmalloc(&ens_eps_d_, (size_t)nc), "ensemble softening");
+  }
+  HIPCHK(hipMemcpyAsync(ens_items_d_, ens_items_.data(), sizeof(EnsItem) * nitems, hipMemcpyHostToDevice, stream_),
+         "work table upload");
+  HIPCHK(hipMemcpyAsync(idesc_d_, ens_idesc_.data(), sizeof(int4) * ni, hipMemcpyHostToDevice, stream_),
+         "descriptor upload");
+  HIPCHK(hipMemcpyAsync(ens_eps_d_, eps2c, sizeof(double) * nc, hipMemcpyHostToDevice, stream_), "eps upload");
+
+  int nupd = 0;
+  if (upload_updates(&nupd) != 0) return -1;
+  if (timing_) HIPCHK(hipEventRecord(ev_d0_, stream_), "event");
+  if (scatter_updates(nupd, ngb_on ? ni : 0) != 0) return -1;
+  if (predict_) {
+    if (predict_range(0, cl_total_) != 0) return -1;
+  } else {
+    timed_predict_ = false;
+  }
+  predict_ = false;
+  if (ensure_partials((size_t)nitems * 64) != 0) return -1;
+  EnsArgs ea{};
+  ea.rec = st_.rec;
+  ea.idr = st_.idr;
+  ea.ipos = ibuf_d_;
+  ea.ivel = ibuf_d_ + 3 * (size_t)ni;
+  ea.iacc = ibuf_d_ + 6 * (size_t)ni;
+  ea.ih2 = ibuf_d_ + (bh - bx);
+  ea.iid = iid_d_;
+  ea.eps2 = ens_eps_d_;
+  ea.items = ens_items_d_;
+  ea.nitems = nitems;
+  ea.idesc = idesc_d_;
+  ea.part = part_d_;
+  ea.part_ds2 = pds2_d_;
+  ea.part_addr = paddr_d_;
+  ea.ngb_cnt = cnt_d_;
+  ea.ngb_list = list_d_;
+  ea.ngb_on = ngb_on ? 1 : 0;
+  FinalizeArgs fz{};
+  fz.ni = ni;
+  fz.rec = st_.rec;
+  fz.ngb_cnt = cnt_d_;
+  fz.ngb_list = list_d_;
+  fz.out = out_d_;
+  fz.ngb_on = ea.ngb_on;
+  const int nwaves = std::min<long>(nitems, (long)cus_ * 4 * 8);
+  if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
+  if (launch_force_ens(stream_, order_, f64_, ea, fz, nwaves) != 0)
+    return fail("ensemble force launch", hipGetLastError());
+  if (timing_) {
+    HIPCHK(hipEventRecord(ev_f1_, stream_), "event");
+    HIPCHK(hipEventRecord(ev_d1_, stream_), "event");
+  }
+  HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
+                        hipMemcpyDeviceToHost, stream_), "result download");
+  HIPCHK(hipEventRecord(ev_done_, stream_), "event");
+  inflight_ = true;
+  have_results_ = true;
+  ngb_fetched_ = false;
+  ngb_on_last_ = ngb_on;
+  ni_last_ = ni;
+  nj_last_ = cl_total_;
+  last_cfg_ = ForceConfig{0, nitems, (int)jch, 0};
+  jhost_valid_ = false;
+  return 0;
 }
 
 // Reference startGravCalc (sapporohostclass.cpp:586-703).
@@ -419,6 +685,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   if (ni == 0 || nj == 0) return 0;  // reference 597
   if (ni < 0 || nj < 0) return fail("start_grav_calc: negative count");
   if (ni > NPIPES) return fail("start_grav_calc: ni exceeds n_pipes");  // reference assert, 604
+  if (ensure_icap(NPIPES) != 0) return -1;
   HIPCHK(hipSetDevice(device_), "hipSetDevice");
   if (inflight_ && wait_results() != 0) return -1;
   if (ensure_store(nj) != 0) return -1;
@@ -456,10 +723,11 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   }
   predict_ = false;  // reference 698
 
-  const ForceConfig cfg = choose_config(ni, nj);
+  const ForceConfig cfg = choose_config(ni, nj, ngb_on);
   if (ensure_partials((size_t)cfg.S * ni) != 0) return -1;
   ForceArgs fa{};
   fa.rec = st_.rec;
+  fa.idr = st_.idr;
   fa.nj = nj;
   fa.ipos = ibuf_d_;
   fa.ivel = ibuf_d_ + 3 * (size_t)ni;

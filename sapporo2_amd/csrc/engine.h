@@ -22,6 +22,7 @@ namespace sap {
 
 constexpr int NPIPES = 16384;        // reference defines.h:13-15
 constexpr int NJ_INITIAL = 16384;    // reference sapporohostclass.cpp:161-162
+constexpr int ENS_MAX_I = 1 << 22;   // i-particles per ensemble call
 
 class Engine {
  public:
@@ -52,6 +53,13 @@ class Engine {
                       const double (*ai)[3], double eps2, const double* h2);
   int get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3], double (*snp)[3],
                        double (*crk)[3], double* pot, int* nnb, double* dsmin, bool ngb);
+  // Ensembles of independent clusters (no reference counterpart).
+  int set_clusters(int n, const int* nj_per_cluster);
+  int cluster_offset(int c) const;
+  int num_clusters() const { return (int)cl_off_.size(); }
+  int start_grav_calc_clusters(int ni, const int* cluster, const int* id, const double (*xi)[3],
+                               const double (*vi)[3], const double (*ai)[3], const double* eps2_per_cluster,
+                               const double* h2);
   int read_ngb_list();
   int get_ngb_list(int ipipe, int maxlen, int* n, int* list);
   int force_prediction(int nj);
@@ -63,7 +71,13 @@ class Engine {
 
   // Launch-shape control and instrumentation (not part of the reference API).
   void set_force_override(int P, int S) { ovr_P_ = P; ovr_S_ = S; }
-  sapk::ForceConfig choose_config(int ni, int nj) const;
+  void set_split_major(bool on) { ovr_smajor_ = on ? 1 : 0; }
+  sapk::ForceConfig choose_config(int ni, int nj, bool ngb) const;
+  // Launch-shape table written by sapporo2_amd/autotune.py: lines
+  // "order precision ngb log2(ni bucket) log2(nj) P log2(j per split)".
+  int load_tuning(const char* path);
+  int tuning_entries() const { return (int)tune_.size(); }
+  const char* tuning_path() const { return tune_path_.c_str(); }
   void set_timing(bool on) { timing_ = on; }
   float last_force_ms();          // force kernel time of the last call (timing on)
   float last_predict_ms();        // predictor time of the last call (timing on, 0 if none)
@@ -86,6 +100,7 @@ class Engine {
   int wait_results();
   int fetch_j_host(int nj);
   int grow_staging(int need);
+  int ensure_icap(int need);
 
   bool open_ = false;
   int device_ = 0, order_ = sapk::ORDER_H4, cus_ = 256;
@@ -130,9 +145,23 @@ class Engine {
   bool inflight_ = false, have_results_ = false, ngb_fetched_ = false, ngb_on_last_ = false;
   bool timed_predict_ = false;
   int ni_last_ = 0, nj_last_ = 0;
-  sapk::ForceConfig last_cfg_{64, 1, 1};
+  sapk::ForceConfig last_cfg_{64, 1, 1, 0};
   bool timing_ = false;
-  int ovr_P_ = 0, ovr_S_ = 0;
+  int ovr_P_ = 0, ovr_S_ = 0, ovr_smajor_ = 0;
+
+  // ensembles
+  int icap_ = 0;
+  std::vector<int> cl_off_, cl_n_;
+  int cl_total_ = 0;
+  std::vector<sapk::EnsItem> ens_items_;
+  std::vector<int4> ens_idesc_;
+  sapk::EnsItem* ens_items_d_ = nullptr; int ens_items_cap_ = 0;
+  int4* idesc_d_ = nullptr;
+  double* ens_eps_d_ = nullptr; int ens_eps_cap_ = 0;
+
+  struct TuneEntry { int order, prec, ngb, li, lj, P, jlog; };
+  std::vector<TuneEntry> tune_;
+  std::string tune_path_, arch_;
 
   // read-back cache of the j-store (predicted and raw state)
   bool jhost_valid_ = false;

@@ -52,13 +52,19 @@ constexpr int out_fields(int order) { return order >= ORDER_H6 ? 13 : 10; }
 constexpr int part_fields(int order) { return order >= ORDER_H6 ? 10 : 7; }
 
 // ---------------------------------------------------------------------------
+// Addresses are grouped in tiles of JTILE; per tile the store keeps a
+// conservative [min id, max id] range so the force kernel can skip the
+// per-pair self-interaction test on tiles that cannot hold an i-particle's id.
+constexpr int JTILE = 64;
+
 struct StoreView {
   double* raw;       // RF_COUNT (or raw_fields) x cap doubles
   int* id;           // cap ints
   void* rec;         // cap predicted records
   double* pvel;      // mixed precision only: 3 x cap FP64 predicted velocity
   double* pacc;      // mixed precision SIXTH only: 3 x cap FP64 predicted acc
-  int cap;
+  int2* idr;         // cap / JTILE tile id ranges (x = min, y = max)
+  int cap;           // multiple of JTILE
 };
 
 // Scatter staged updates into the store and reset the per-i neighbour
@@ -80,6 +86,7 @@ struct PredictArgs {
 
 struct ForceArgs {
   const void* rec;       // predicted records
+  const int2* idr;       // tile id ranges (StoreView::idr)
   int nj;
   const double* ipos;    // ni x 3
   const double* ivel;    // ni x 3
@@ -98,7 +105,7 @@ struct ForceArgs {
 
 // Launch geometry of one force call: each wave owns P i-particles and Q = 64/P
 // j-lanes per i-particle over one of S contiguous j-ranges of jchunk.
-struct ForceConfig { int P; int S; int jchunk; };
+struct ForceConfig { int P; int S; int jchunk; int smajor; };
 
 struct FinalizeArgs {
   const double* part; const double* part_ds2; const int* part_addr;
@@ -107,6 +114,20 @@ struct FinalizeArgs {
   int* ngb_cnt; int* ngb_list;
   double* out;           // ni x out_fields
   int ngb_on;
+};
+
+// Ensemble work item: i-particles [i0, i0 + icount) of one cluster (icount <=
+// P, P in {64, 16, 4, 1}) against that cluster's j in [jb, je).
+struct EnsItem { int i0, icount, jb, je, P, cluster, pad0, pad1; };
+
+struct EnsArgs {
+  const void* rec; const int2* idr;
+  const double* ipos; const double* ivel; const double* iacc; const double* ih2; const int* iid;
+  const double* eps2;        // per cluster
+  const EnsItem* items; int nitems;
+  const int4* idesc;         // per i: first item, number of splits, i-slot
+  double* part; double* part_ds2; int* part_addr;   // part_fields x nitems x 64, nitems x 64
+  int* ngb_cnt; int* ngb_list; int ngb_on;
 };
 
 struct PackArgs {
@@ -124,6 +145,8 @@ int launch_predict(hipStream_t s, int order, bool f64, const PredictArgs& a);
 int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const ForceConfig& c);
 int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a);
 int launch_pack(hipStream_t s, const PackArgs& a);
+// Ensemble force + finalize (+ list sort) over a work table; nwaves persistent waves.
+int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int nwaves);
 int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a);
 
 // The P values with compiled force kernels.

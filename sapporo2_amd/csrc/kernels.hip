@@ -68,6 +68,9 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     if (addr < 0 || addr >= a.st.cap) continue;  // host guarantees range
     for (int f = 0; f < a.nf; ++f) a.st.raw[(size_t)f * cap + addr] = u[f];
     a.st.id[addr] = id;
+    // widen the tile's id range (conservative until the next prediction recomputes it)
+    atomicMin(&a.st.idr[addr / JTILE].x, id);
+    atomicMax(&a.st.idr[addr / JTILE].y, id);
     // predicted = raw for updated particles (kernels.cu:281-298)
     double ax = 0, ay = 0, az = 0;
     if constexpr (ORDER >= ORDER_H6) { ax = u[RF_AX]; ay = u[RF_AY]; az = u[RF_AZ]; }
@@ -112,6 +115,18 @@ __device__ __forceinline__ void predict_one(const double* __restrict__ R, size_t
 template <int ORDER, bool F64>
 __global__ __launch_bounds__(256) void k_predict(PredictArgs a) {
   const int j = a.j0 + blockIdx.x * blockDim.x + threadIdx.x;
+  // exact id range of this wave's tile (j0 is a multiple of JTILE = wave size);
+  // addresses past j1 are included so later calls with a larger nj stay covered
+  {
+    const int id = j < a.st.cap ? a.st.id[j] : 0;
+    int lo = j < a.st.cap ? id : INT_MAX, hi = j < a.st.cap ? id : INT_MIN;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      lo = min(lo, __shfl_xor(lo, m));
+      hi = max(hi, __shfl_xor(hi, m));
+    }
+    if ((threadIdx.x & 63) == 0 && j < a.st.cap) a.st.idr[j / JTILE] = make_int2(lo, hi);
+  }
   if (j >= a.j1) return;
   const size_t cap = (size_t)a.st.cap;
   double p[3], v[3], ac[3];
@@ -157,15 +172,15 @@ constexpr int NF4 = 7, NF6 = 10;
 template <int ORDER> constexpr int nf() { return ORDER >= ORDER_H6 ? NF6 : NF4; }
 
 // One pair interaction (kernels.cu:565-667) accumulated into t[].
-template <int ORDER, bool NGB, typename R, typename Rec>
-__device__ __forceinline__ void pair(const Rec& r, const int j, const bool ok, const IState<ORDER, R>& s,
+template <int ORDER, bool NGB, bool CHECK, typename R, typename Rec>
+__device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, const IState<ORDER, R>& s,
                                      const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
                                      int* __restrict__ ngb_cnt, int* __restrict__ ngb_list) {
   const R dx = to_rel<R>(r.x, s.x);
   const R dy = to_rel<R>(r.y, s.y);
   const R dz = to_rel<R>(r.z, s.z);
   const R ds2 = fmaR<R>(dz, dz, fmaR<R>(dy, dy, dx * dx));
-  const bool use = ok && (r.id != s.id);
+  const bool use = CHECK ? (ok && (r.id != s.id)) : ok;
   const R inv0 = rsq<R>(ds2 + eps2);
   const R inv = use ? inv0 : R(0);
   const R invr2 = inv * inv;
@@ -213,41 +228,62 @@ __device__ __forceinline__ void pair(const Rec& r, const int j, const bool ok, c
   const bool better = use && (ds2 < dsmin);
   dsmin = better ? ds2 : dsmin;
   nnaddr = better ? j : nnaddr;
-  // neighbour list: h2_i > ds2 (kernels.cu:584); rare, so a global atomic slot
+  // neighbour predicate h2_i > ds2 (kernels.cu:584); the caller appends
+  return NGB && use && (s.h2 > ds2);
+}
+
+// Append id to the i-particle's list; rare (~32 per 1e5 pairs), so a global
+// atomic takes the slot; the count stays exact past NGB_CAP.
+__device__ __forceinline__ void ngb_append(int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, int i, int id) {
+  const int k = atomicAdd(&ngb_cnt[i], 1);
+  if (k < NGB_CAP) ngb_list[(size_t)i * NGB_CAP + k] = id;
+}
+
+// Four consecutive j of one lane with a single neighbour branch for the group.
+template <int ORDER, bool NGB, bool CHECK, int Q, typename R, typename Rec>
+__device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const IState<ORDER, R>& s,
+                                      const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
+                                      int* __restrict__ ngb_cnt, int* __restrict__ ngb_list) {
+  bool nb[4];
+  int idj[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int j = j0 + u * Q;
+    const Rec& r = rec[j];
+    idj[u] = r.id;
+    nb[u] = pair<ORDER, NGB, CHECK>(r, j, true, s, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+  }
   if constexpr (NGB) {
-    if (use && (s.h2 > ds2)) {
-      const int k = atomicAdd(&ngb_cnt[s.i], 1);
-      if (k < NGB_CAP) ngb_list[(size_t)s.i * NGB_CAP + k] = r.id;
+    if (nb[0] | nb[1] | nb[2] | nb[3]) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (nb[u]) ngb_append(ngb_cnt, ngb_list, s.i, idj[u]);
     }
   }
 }
 
 constexpr int KT = 64;  // interactions per lane between FP64 folds
 
+// The work of one wave: i-particles [i0, i0 + icount) (icount <= P) against
+// j in [jb, je), Q = 64/P j-lanes per i-particle.  On return the lanes with
+// writer set hold the wave-reduced FP64 sums T, nearest-neighbour (ds2,
+// address) and their i-slot.
 template <int ORDER, bool F64, int P, bool NGB>
-__global__ __launch_bounds__(256) void k_force(
-    const typename RecOf<ORDER, F64>::T* __restrict__ rec, const int nj,
-    const double* __restrict__ ipos, const double* __restrict__ ivel,
-    const double* __restrict__ iacc, const double* __restrict__ ih2,
-    const int* __restrict__ iid, const int ni, const double eps2d,
-    double* __restrict__ part, double* __restrict__ part_ds2, int* __restrict__ part_addr,
-    int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, const int S, const int jchunk) {
+__device__ __forceinline__ void wave_force(
+    const typename RecOf<ORDER, F64>::T* __restrict__ rec, const int2* __restrict__ idr,
+    const double* __restrict__ ipos, const double* __restrict__ ivel, const double* __restrict__ iacc,
+    const double* __restrict__ ih2, const int* __restrict__ iid, const int i0, const int icount,
+    const int jb, const int je, const double eps2d, int* __restrict__ ngb_cnt, int* __restrict__ ngb_list,
+    double (&T)[nf<ORDER>()], double& dsd, int& nnaddr, int& islot, bool& writer) {
   using R = typename std::conditional<F64, double, float>::type;
+  using Rec = typename RecOf<ORDER, F64>::T;
   constexpr int Q = 64 / P;
   constexpr int NF = nf<ORDER>();
   const int lane = threadIdx.x & 63;
-  // wave-uniform by construction; readfirstlane lets the compiler keep the
-  // j-loop bookkeeping and (for P = 64) the j-record loads in SGPRs
-  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  const int G = (ni + P - 1) / P;
-  if (wave >= G * S) return;  // whole wave
-  const int g = wave / S;
-  const int sidx = wave - g * S;
-  const int islot = (Q == 1) ? lane : (lane % P);
+  islot = (Q == 1) ? lane : (lane % P);
   const int jsub = (Q == 1) ? 0 : (lane / P);
-  const int i = g * P + islot;
-  const bool ivalid = i < ni;
-  const int il = ivalid ? i : ni - 1;
+  const bool ivalid = islot < icount;
+  const int il = i0 + (ivalid ? islot : 0);
 
   IState<ORDER, R> st;
   st.x = ipos[3 * il]; st.y = ipos[3 * il + 1]; st.z = ipos[3 * il + 2];
@@ -261,14 +297,11 @@ __global__ __launch_bounds__(256) void k_force(
   st.i = il;
   const R eps2 = (R)eps2d;
 
-  double T[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) T[f] = 0.0;
   R dsmin = R(INFINITY);
-  int nnaddr = INT_MAX;
+  nnaddr = INT_MAX;
 
-  const int jb = sidx * jchunk;
-  const int je = min(nj, jb + jchunk);
   const int nk = (je > jb) ? (je - jb + Q - 1) / Q : 0;  // iterations of the lane with jsub 0
   for (int k0 = 0; k0 < nk; k0 += KT) {
     R t[NF];
@@ -277,23 +310,36 @@ __global__ __launch_bounds__(256) void k_force(
     const int kn = min(KT, nk - k0);
     const int jt = jb + k0 * Q + jsub;
     if (kn == KT && jb + (k0 + KT) * Q <= je) {
-#pragma unroll 4
-      for (int k = 0; k < KT; ++k) {
-        const int j = jt + k * Q;
-        pair<ORDER, NGB>(rec[j], j, true, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+      // P = 64: one JTILE-aligned tile per fold; skip the self test when no
+      // lane's id can occur in the tile
+      bool check = true;
+      if constexpr (Q == 1) {
+        if ((jt & (JTILE - 1)) == 0) {
+          const int2 r = idr[jt / JTILE];
+          check = __ballot(st.id >= r.x && st.id <= r.y) != 0;
+        }
+      }
+      if (check) {
+        for (int k = 0; k < KT; k += 4)
+          pair4<ORDER, NGB, true, Q>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+      } else {
+        for (int k = 0; k < KT; k += 4)
+          pair4<ORDER, NGB, false, Q>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
       }
     } else {
       for (int k = 0; k < kn; ++k) {
         const int j = jt + k * Q;
         const bool ok = j < je;
-        pair<ORDER, NGB>(rec[ok ? j : jb], j, ok, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+        const Rec& r = rec[ok ? j : jb];
+        if (pair<ORDER, NGB, true>(r, j, ok, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list))
+          ngb_append(ngb_cnt, ngb_list, st.i, r.id);
       }
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) T[f] += (double)t[f];
   }
 
-  double dsd = (double)dsmin;
+  dsd = (double)dsmin;
   if constexpr (Q > 1) {
 #pragma unroll
     for (int m = P; m < 64; m <<= 1) {
@@ -306,7 +352,37 @@ __global__ __launch_bounds__(256) void k_force(
       nnaddr = take ? oa : nnaddr;
     }
   }
-  if (ivalid && jsub == 0) {
+  writer = ivalid && jsub == 0;
+}
+
+template <int ORDER, bool F64, int P, bool NGB>
+__global__ __launch_bounds__(256) void k_force(
+    const typename RecOf<ORDER, F64>::T* __restrict__ rec, const int2* __restrict__ idr, const int nj,
+    const double* __restrict__ ipos, const double* __restrict__ ivel,
+    const double* __restrict__ iacc, const double* __restrict__ ih2,
+    const int* __restrict__ iid, const int ni, const double eps2d,
+    double* __restrict__ part, double* __restrict__ part_ds2, int* __restrict__ part_addr,
+    int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, const int S, const int jchunk, const int smajor) {
+  constexpr int NF = nf<ORDER>();
+  // wave-uniform by construction; readfirstlane lets the compiler keep the
+  // j-loop bookkeeping and (for P = 64) the j-record loads in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int G = (ni + P - 1) / P;
+  if (wave >= G * S) return;  // whole wave
+  // smajor: consecutive waves (a block's 4) share one j-range and so its
+  // scalar-cache lines; otherwise consecutive waves walk the splits of one group
+  const int g = smajor ? wave % G : wave / S;
+  const int sidx = smajor ? wave / G : wave - g * S;
+  const int jb = sidx * jchunk;
+  const int je = min(nj, jb + jchunk);
+  double T[NF];
+  double dsd;
+  int nnaddr, islot;
+  bool writer;
+  wave_force<ORDER, F64, P, NGB>(rec, idr, ipos, ivel, iacc, ih2, iid, g * P, min(P, ni - g * P), jb, je, eps2d,
+                                 ngb_cnt, ngb_list, T, dsd, nnaddr, islot, writer);
+  if (writer) {
+    const int i = g * P + islot;
 #pragma unroll
     for (int f = 0; f < NF; ++f) part[((size_t)f * S + sidx) * ni + i] = T[f];
     part_ds2[(size_t)sidx * ni + i] = dsd;
@@ -314,72 +390,213 @@ __global__ __launch_bounds__(256) void k_force(
   }
 }
 
-// ---------------------------------------------------------------------------
-// Finalize: block of 4 waves handles 4 i-particles (one per wave).
 template <int ORDER, bool F64>
-__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) {
+__device__ __forceinline__ void write_row(const FinalizeArgs& a, int i, const double* T, double ds, int ad,
+                                          int cnt);
+
+// Ensemble force: a persistent grid of waves walks a work table of items
+// (cluster, i-group, j-split) round-robin; each item carries its own lane shape.
+template <int ORDER, bool F64, int P, bool NGB>
+__device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, int item) {
+  constexpr int NF = nf<ORDER>();
+  using Rec = typename RecOf<ORDER, F64>::T;
+  double T[NF];
+  double dsd;
+  int nnaddr, islot;
+  bool writer;
+  wave_force<ORDER, F64, P, NGB>(reinterpret_cast<const Rec*>(a.rec), a.idr, a.ipos, a.ivel, a.iacc, a.ih2, a.iid,
+                                 it.i0, it.icount, it.jb, it.je, a.eps2[it.cluster], a.ngb_cnt, a.ngb_list, T,
+                                 dsd, nnaddr, islot, writer);
+  if (writer) {
+    const size_t row = (size_t)item * 64 + islot;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) a.part[(size_t)f * a.nitems * 64 + row] = T[f];
+    a.part_ds2[row] = dsd;
+    a.part_addr[row] = nnaddr;
+  }
+}
+
+template <int ORDER, bool F64, bool NGB>
+__global__ __launch_bounds__(256) void k_force_ens(EnsArgs a) {
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  for (int item = wave; item < a.nitems; item += nwaves) {  // every wave leaves after its last item
+    const EnsItem it = a.items[item];
+    switch (it.P) {
+      case 64: ens_item<ORDER, F64, 64, NGB>(a, it, item); break;
+      case 16: ens_item<ORDER, F64, 16, NGB>(a, it, item); break;
+      case 4: ens_item<ORDER, F64, 4, NGB>(a, it, item); break;
+      default: ens_item<ORDER, F64, 1, NGB>(a, it, item); break;
+    }
+  }
+}
+
+// Ensemble finalize, thread per i-particle: fixed-order sum over its items.
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_finalize_ens(FinalizeArgs f, EnsArgs a) {
+  constexpr int NF = nf<ORDER>();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= f.ni) return;
+  const int4 d = a.idesc[i];  // first item, number of splits, i-slot
+  double T[NF];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) T[q] = 0.0;
+  double ds = INFINITY;
+  int ad = INT_MAX;
+  for (int s = 0; s < d.y; ++s) {
+    const size_t row = (size_t)(d.x + s) * 64 + d.z;
+#pragma unroll
+    for (int q = 0; q < NF; ++q) T[q] += a.part[(size_t)q * a.nitems * 64 + row];
+    const double dd = a.part_ds2[row];
+    const int ad2 = a.part_addr[row];
+    const bool take = (dd < ds) || (dd == ds && ad2 < ad);
+    ds = take ? dd : ds;
+    ad = take ? ad2 : ad;
+  }
+  write_row<ORDER, F64>(f, i, T, ds, ad, f.ngb_on ? f.ngb_cnt[i] : 0);
+}
+
+// ---------------------------------------------------------------------------
+// Finalize.  Fixed-order sums of the S partial rows of each i-particle, the
+// nearest neighbour as the (ds2, address) minimum, the result row, and the
+// ascending sort of the neighbour list.  Two shapes:
+//   k_finalize_rows  one thread per i-particle (partials read coalesced over i),
+//                    followed by k_sort_lists (one wave per list);
+//   k_finalize_block one work-group per i-particle, threads striding over the
+//                    splits (few i-particles, many splits).
+template <int ORDER, bool F64>
+__device__ __forceinline__ void write_row(const FinalizeArgs& a, int i, const double* T, double ds, int ad,
+                                          int cnt) {
   using Rec = typename RecOf<ORDER, F64>::T;
   constexpr int NF = nf<ORDER>();
   constexpr int NO = out_fields(ORDER);
-  __shared__ int lst[4][NGB_CAP];
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int i = blockIdx.x * 4 + w;
-  const bool active = i < a.ni;
-  int c = 0;
-  if (active) {
-    double T[NF];
+  double* o = a.out + (size_t)i * NO;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) T[f] = 0.0;
-    double ds = INFINITY;
-    int ad = INT_MAX;
-    for (int s = lane; s < a.S; s += 64) {
+  for (int f = 0; f < NF; ++f) o[f] = T[f];
+  const bool has = ad != INT_MAX;
+  // the device returns ds2 rounded to float (kernels.cu:999-1002)
+  o[NF] = has ? (double)(float)ds : INFINITY;
+  o[NF + 1] = has ? (double)reinterpret_cast<const Rec*>(a.rec)[ad].id : -1.0;
+  o[NF + 2] = (double)cnt;
+}
+
+// Rank sort of c ids staged in s[0..c) by one wave, written to dst.
+__device__ __forceinline__ void wave_rank_sort(const int* s, int c, int lane, int* dst) {
+  for (int k = lane; k < c; k += 64) {
+    const int v = s[k];
+    int rank = 0;
+    for (int q = 0; q < c; ++q) {
+      const int u = s[q];
+      rank += (u < v) || (u == v && q < k);
+    }
+    dst[rank] = v;
+  }
+}
+
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
+  constexpr int NF = nf<ORDER>();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.ni) return;
+  double T[NF];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * a.ni + i];
-      const double d = a.part_ds2[(size_t)s * a.ni + i];
-      const int ad2 = a.part_addr[(size_t)s * a.ni + i];
+  for (int f = 0; f < NF; ++f) T[f] = 0.0;
+  double ds = INFINITY;
+  int ad = INT_MAX;
+  // groups of 4 splits summed as ((p0 + p1) + (p2 + p3)): fixed order, 4x the loads in flight
+  int s = 0;
+  for (; s + 4 <= a.S; s += 4) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const double* p = a.part + ((size_t)f * a.S + s) * a.ni + i;
+      T[f] += (p[0] + p[a.ni]) + (p[2 * (size_t)a.ni] + p[3 * (size_t)a.ni]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double d = a.part_ds2[(size_t)(s + u) * a.ni + i];
+      const int ad2 = a.part_addr[(size_t)(s + u) * a.ni + i];
       const bool take = (d < ds) || (d == ds && ad2 < ad);
       ds = take ? d : ds;
       ad = take ? ad2 : ad;
     }
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-#pragma unroll
-      for (int f = 0; f < NF; ++f) T[f] += __shfl_xor(T[f], m);
-      const double od = __shfl_xor(ds, m);
-      const int oa = __shfl_xor(ad, m);
-      const bool take = (od < ds) || (od == ds && oa < ad);
-      ds = take ? od : ds;
-      ad = take ? oa : ad;
-    }
-    const int cnt = a.ngb_on ? a.ngb_cnt[i] : 0;
-    if (lane == 0) {
-      double* o = a.out + (size_t)i * NO;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) o[f] = T[f];
-      const bool has = ad != INT_MAX;
-      // the device returns ds2 rounded to float (kernels.cu:999-1002)
-      o[NF] = has ? (double)(float)ds : INFINITY;
-      o[NF + 1] = has ? (double)reinterpret_cast<const Rec*>(a.rec)[ad].id : -1.0;
-      o[NF + 2] = (double)cnt;
-    }
-    c = min(cnt, NGB_CAP);
-    if (c > 1)
-      for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
   }
+  for (; s < a.S; ++s) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * a.ni + i];
+    const double d = a.part_ds2[(size_t)s * a.ni + i];
+    const int ad2 = a.part_addr[(size_t)s * a.ni + i];
+    const bool take = (d < ds) || (d == ds && ad2 < ad);
+    ds = take ? d : ds;
+    ad = take ? ad2 : ad;
+  }
+  write_row<ORDER, F64>(a, i, T, ds, ad, a.ngb_on ? a.ngb_cnt[i] : 0);
+}
+
+// One wave per i-particle: ascending sort of its (at most NGB_CAP) list.
+__global__ __launch_bounds__(256) void k_sort_lists(FinalizeArgs a) {
+  __shared__ int lst[4][NGB_CAP];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 4 + w;
+  const int c = i < a.ni ? min(a.ngb_cnt[i], NGB_CAP) : 0;
+  if (c > 1)
+    for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
   __syncthreads();
-  if (active && c > 1) {
-    // rank sort (ascending ids, stable for equal ids)
-    for (int k = lane; k < c; k += 64) {
-      const int v = lst[w][k];
-      int rank = 0;
-      for (int q = 0; q < c; ++q) {
-        const int u = lst[w][q];
-        rank += (u < v) || (u == v && q < k);
-      }
-      a.ngb_list[(size_t)i * NGB_CAP + rank] = v;
-    }
+  if (c > 1) wave_rank_sort(lst[w], c, lane, a.ngb_list + (size_t)i * NGB_CAP);
+}
+
+// One 256-thread block per i-particle: thread t sums the splits s = t, t+256, ...
+// in order, then a fixed-order tree over the 256 threads in LDS (deterministic).
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_finalize_block(FinalizeArgs a) {
+  constexpr int NF = nf<ORDER>();
+  __shared__ double shT[NF][256];
+  __shared__ double shD[256];
+  __shared__ int shA[256];
+  __shared__ int lst[NGB_CAP];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x;
+  double T[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) T[f] = 0.0;
+  double ds = INFINITY;
+  int ad = INT_MAX;
+  for (int s = t; s < a.S; s += 256) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * a.ni + i];
+    const double d = a.part_ds2[(size_t)s * a.ni + i];
+    const int ad2 = a.part_addr[(size_t)s * a.ni + i];
+    const bool take = (d < ds) || (d == ds && ad2 < ad);
+    ds = take ? d : ds;
+    ad = take ? ad2 : ad;
   }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) shT[f][t] = T[f];
+  shD[t] = ds;
+  shA[t] = ad;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) shT[f][t] += shT[f][t + h];
+      const double od = shD[t + h];
+      const int oa = shA[t + h];
+      if ((od < shD[t]) || (od == shD[t] && oa < shA[t])) { shD[t] = od; shA[t] = oa; }
+    }
+    __syncthreads();
+  }
+  const int cnt = a.ngb_on ? a.ngb_cnt[i] : 0;
+  if (t == 0) {
+    double R[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) R[f] = shT[f][0];
+    write_row<ORDER, F64>(a, i, R, shD[0], shA[0], cnt);
+  }
+  const int c = min(cnt, NGB_CAP);
+  if (c <= 1) return;  // uniform over the block
+  for (int k = t; k < c; k += 256) lst[k] = a.ngb_list[(size_t)i * NGB_CAP + k];
+  __syncthreads();
+  if (t < 64) wave_rank_sort(lst, c, t, a.ngb_list + (size_t)i * NGB_CAP);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,13 +706,13 @@ static void launch_force_p(hipStream_t s, const ForceArgs& a, const ForceConfig&
   const int nb = blocks_for(G * c.S, 4);
   const Rec* rec = reinterpret_cast<const Rec*>(a.rec);
   if (a.ngb_on)
-    hipLaunchKernelGGL((k_force<OR, FP, P, true>), dim3(nb), dim3(256), 0, s, rec, a.nj, a.ipos, a.ivel,
+    hipLaunchKernelGGL((k_force<OR, FP, P, true>), dim3(nb), dim3(256), 0, s, rec, a.idr, a.nj, a.ipos, a.ivel,
                        a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, a.part_addr, a.ngb_cnt,
-                       a.ngb_list, c.S, c.jchunk);
+                       a.ngb_list, c.S, c.jchunk, c.smajor);
   else
-    hipLaunchKernelGGL((k_force<OR, FP, P, false>), dim3(nb), dim3(256), 0, s, rec, a.nj, a.ipos, a.ivel,
+    hipLaunchKernelGGL((k_force<OR, FP, P, false>), dim3(nb), dim3(256), 0, s, rec, a.idr, a.nj, a.ipos, a.ivel,
                        a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, a.part_addr, a.ngb_cnt,
-                       a.ngb_list, c.S, c.jchunk);
+                       a.ngb_list, c.S, c.jchunk, c.smajor);
 }
 
 template <int OR, bool FP>
@@ -523,9 +740,33 @@ int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const F
 
 int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a) {
   if (a.ni <= 0) return 0;
-  const int nb = blocks_for(a.ni, 4);
+  // rows: coalesced over i, best when many i-particles share few splits
+  const bool rows = a.ni >= 1024;
   dispatch_op(order, f64, [&](auto OR, auto FP) {
-    hipLaunchKernelGGL((k_finalize<decltype(OR)::value, decltype(FP)::value>), dim3(nb), dim3(256), 0, s, a);
+    constexpr int O = decltype(OR)::value;
+    constexpr bool F = decltype(FP)::value;
+    if (rows) {
+      hipLaunchKernelGGL((k_finalize_rows<O, F>), dim3(blocks_for(a.ni, 256)), dim3(256), 0, s, a);
+      if (a.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(a.ni, 4)), dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_finalize_block<O, F>), dim3(a.ni), dim3(256), 0, s, a);
+    }
+  });
+  return hip_status();
+}
+
+int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int nwaves) {
+  if (a.nitems <= 0 || f.ni <= 0) return 0;
+  const int nb = blocks_for(nwaves, 4);
+  dispatch_op(order, f64, [&](auto OR, auto FP) {
+    constexpr int O = decltype(OR)::value;
+    constexpr bool F = decltype(FP)::value;
+    if (a.ngb_on)
+      hipLaunchKernelGGL((k_force_ens<O, F, true>), dim3(nb), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_force_ens<O, F, false>), dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_finalize_ens<O, F>), dim3(blocks_for(f.ni, 256)), dim3(256), 0, s, f, a);
+    if (f.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(f.ni, 4)), dim3(256), 0, s, f);
   });
   return hip_status();
 }

@@ -300,6 +300,23 @@ def extra_workloads(log):
     return out
 
 
+def copy_calibration(mb):
+    """Three device-to-device copies of `mb` MiB through the HIP runtime: the known-size
+    reference that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mb) << 20
+    a = ctypes.c_void_p(); b = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a); hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,6 +325,8 @@ def main():
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / accuracy leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the other configurations")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run the known-size copy used to calibrate traffic counters")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="run one kernel only (profiling)")
     args = ap.parse_args()
@@ -315,6 +334,9 @@ def main():
         print(f"bench.py: this engine drives one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU j-split is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     steps, warmup = max(1, args.steps), max(0, args.warmup)
     t_setup = time.perf_counter()
     w = Workload(args.n, SEED)

@@ -93,6 +93,9 @@ int sapamd_set_force_override(sapamd_engine *e, int P, int S);
 /* on: the 4 waves of a work-group share one j-range; off (default, faster on
  * MI355X): they take consecutive j-splits of one i-group */
 int sapamd_set_split_major(sapamd_engine *e, int on);
+/* on: neighbour-id stores are deferred by one group of pairs so the slot
+ * atomic's latency overlaps arithmetic (more registers); off: immediate */
+int sapamd_set_ngb_pipeline(sapamd_engine *e, int on);
 int sapamd_set_timing(sapamd_engine *e, int on);
 float sapamd_last_force_ms(sapamd_engine *e);
 float sapamd_last_predict_ms(sapamd_engine *e);

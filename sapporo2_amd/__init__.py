@@ -51,7 +51,7 @@ ENGINE_SYMBOLS = [
     "sapamd_set_time", "sapamd_set_no_time", "sapamd_set_j_particle", "sapamd_start_grav_calc",
     "sapamd_get_grav_results", "sapamd_set_j_particles", "sapamd_last_device_ms", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
-    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_timing",
+    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing",
     "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
     "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
     "sapamd_start_grav_calc_clusters",
@@ -96,6 +96,7 @@ def lib():
         L.sapamd_set_force_override.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.sapamd_set_timing.argtypes = [_vp, ctypes.c_int]
         L.sapamd_set_split_major.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_set_ngb_pipeline.argtypes = [_vp, ctypes.c_int]
         L.sapamd_last_force_ms.argtypes = [_vp]
         L.sapamd_last_force_ms.restype = ctypes.c_float
         L.sapamd_last_predict_ms.argtypes = [_vp]
@@ -298,6 +299,9 @@ class Engine:
 
     def set_split_major(self, on=True):
         self._L.sapamd_set_split_major(self._h, 1 if on else 0)
+
+    def set_ngb_pipeline(self, on=True):
+        self._L.sapamd_set_ngb_pipeline(self._h, 1 if on else 0)
 
     def set_timing(self, on=True):
         self._L.sapamd_set_timing(self._h, 1 if on else 0)

@@ -90,6 +90,11 @@ int sapamd_set_split_major(sapamd_engine* e, int on) {
   e->eng.set_split_major(on != 0);
   return 0;
 }
+int sapamd_set_ngb_pipeline(sapamd_engine* e, int on) {
+  if (!e) return -1;
+  e->eng.set_ngb_pipeline(on != 0);
+  return 0;
+}
 int sapamd_set_timing(sapamd_engine* e, int on) {
   if (!e) return -1;
   e->eng.set_timing(on != 0);

@@ -471,7 +471,7 @@ ForceConfig Engine::choose_config(int ni, int nj, bool ngb) const {
   if (jchunk < 1) jchunk = 1;
   S = (nj + jchunk - 1) / jchunk;
   if (S < 1) S = 1;
-  return ForceConfig{P, (int)S, (int)jchunk, ovr_smajor_};
+  return ForceConfig{P, (int)S, (int)jchunk, ovr_smajor_, ovr_pipe_};
 }
 
 // i-particle-sized buffers (inputs, result rows, neighbour lists) for up to
@@ -673,7 +673,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ngb_on_last_ = ngb_on;
   ni_last_ = ni;
   nj_last_ = cl_total_;
-  last_cfg_ = ForceConfig{0, nitems, (int)jch, 0};
+  last_cfg_ = ForceConfig{0, nitems, (int)jch, 0, 0};
   jhost_valid_ = false;
   return 0;
 }

@@ -72,6 +72,7 @@ class Engine {
   // Launch-shape control and instrumentation (not part of the reference API).
   void set_force_override(int P, int S) { ovr_P_ = P; ovr_S_ = S; }
   void set_split_major(bool on) { ovr_smajor_ = on ? 1 : 0; }
+  void set_ngb_pipeline(bool on) { ovr_pipe_ = on ? 1 : 0; }
   sapk::ForceConfig choose_config(int ni, int nj, bool ngb) const;
   // Launch-shape table written by sapporo2_amd/autotune.py: lines
   // "order precision ngb log2(ni bucket) log2(nj) P log2(j per split)".
@@ -145,9 +146,9 @@ class Engine {
   bool inflight_ = false, have_results_ = false, ngb_fetched_ = false, ngb_on_last_ = false;
   bool timed_predict_ = false;
   int ni_last_ = 0, nj_last_ = 0;
-  sapk::ForceConfig last_cfg_{64, 1, 1, 0};
+  sapk::ForceConfig last_cfg_{64, 1, 1, 0, 0};
   bool timing_ = false;
-  int ovr_P_ = 0, ovr_S_ = 0, ovr_smajor_ = 0;
+  int ovr_P_ = 0, ovr_S_ = 0, ovr_smajor_ = 0, ovr_pipe_ = 0;
 
   // ensembles
   int icap_ = 0;

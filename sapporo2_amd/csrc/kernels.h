@@ -105,7 +105,8 @@ struct ForceArgs {
 
 // Launch geometry of one force call: each wave owns P i-particles and Q = 64/P
 // j-lanes per i-particle over one of S contiguous j-ranges of jchunk.
-struct ForceConfig { int P; int S; int jchunk; int smajor; };
+// smajor: wave-to-split mapping; pipe: neighbour-id stores deferred by one group.
+struct ForceConfig { int P; int S; int jchunk; int smajor; int pipe; };
 
 struct FinalizeArgs {
   const double* part; const double* part_ds2; const int* part_addr;

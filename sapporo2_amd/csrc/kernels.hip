@@ -239,11 +239,29 @@ __device__ __forceinline__ void ngb_append(int* __restrict__ ngb_cnt, int* __res
   if (k < NGB_CAP) ngb_list[(size_t)i * NGB_CAP + k] = id;
 }
 
+// Neighbour slots reserved by one group whose id stores are still pending.
+struct PendingNgb {
+  int k[4], id[4];
+  bool m[4];
+};
+
+// Store the ids of the previous group (their slot atomics were issued one
+// group earlier, so the atomic latency overlaps this group's arithmetic).
+__device__ __forceinline__ void ngb_flush(PendingNgb& p, int* __restrict__ ngb_list, int i) {
+  if (p.m[0] | p.m[1] | p.m[2] | p.m[3]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (p.m[u] && p.k[u] < NGB_CAP) ngb_list[(size_t)i * NGB_CAP + p.k[u]] = p.id[u];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) p.m[u] = false;
+}
+
 // Four consecutive j of one lane with a single neighbour branch for the group.
-template <int ORDER, bool NGB, bool CHECK, int Q, typename R, typename Rec>
+template <int ORDER, bool NGB, bool CHECK, int Q, bool PIPE, typename R, typename Rec>
 __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const IState<ORDER, R>& s,
                                       const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
-                                      int* __restrict__ ngb_cnt, int* __restrict__ ngb_list) {
+                                      int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, PendingNgb& pend) {
   bool nb[4];
   int idj[4];
 #pragma unroll
@@ -253,7 +271,17 @@ __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0,
     idj[u] = r.id;
     nb[u] = pair<ORDER, NGB, CHECK>(r, j, true, s, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
   }
-  if constexpr (NGB) {
+  if constexpr (NGB && PIPE) {
+    ngb_flush(pend, ngb_list, s.i);
+    if (nb[0] | nb[1] | nb[2] | nb[3]) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (nb[u]) pend.k[u] = atomicAdd(&ngb_cnt[s.i], 1);
+        pend.id[u] = idj[u];
+        pend.m[u] = nb[u];
+      }
+    }
+  } else if constexpr (NGB) {
     if (nb[0] | nb[1] | nb[2] | nb[3]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -268,7 +296,7 @@ constexpr int KT = 64;  // interactions per lane between FP64 folds
 // j in [jb, je), Q = 64/P j-lanes per i-particle.  On return the lanes with
 // writer set hold the wave-reduced FP64 sums T, nearest-neighbour (ds2,
 // address) and their i-slot.
-template <int ORDER, bool F64, int P, bool NGB>
+template <int ORDER, bool F64, int P, bool NGB, bool PIPE = false>
 __device__ __forceinline__ void wave_force(
     const typename RecOf<ORDER, F64>::T* __restrict__ rec, const int2* __restrict__ idr,
     const double* __restrict__ ipos, const double* __restrict__ ivel, const double* __restrict__ iacc,
@@ -301,6 +329,9 @@ __device__ __forceinline__ void wave_force(
   for (int f = 0; f < NF; ++f) T[f] = 0.0;
   R dsmin = R(INFINITY);
   nnaddr = INT_MAX;
+  PendingNgb pend;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) { pend.m[u] = false; pend.k[u] = 0; pend.id[u] = 0; }
 
   const int nk = (je > jb) ? (je - jb + Q - 1) / Q : 0;  // iterations of the lane with jsub 0
   for (int k0 = 0; k0 < nk; k0 += KT) {
@@ -321,12 +352,13 @@ __device__ __forceinline__ void wave_force(
       }
       if (check) {
         for (int k = 0; k < KT; k += 4)
-          pair4<ORDER, NGB, true, Q>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+          pair4<ORDER, NGB, true, Q, PIPE>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list, pend);
       } else {
         for (int k = 0; k < KT; k += 4)
-          pair4<ORDER, NGB, false, Q>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+          pair4<ORDER, NGB, false, Q, PIPE>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list, pend);
       }
     } else {
+      if constexpr (NGB && PIPE) ngb_flush(pend, ngb_list, st.i);
       for (int k = 0; k < kn; ++k) {
         const int j = jt + k * Q;
         const bool ok = j < je;
@@ -339,6 +371,7 @@ __device__ __forceinline__ void wave_force(
     for (int f = 0; f < NF; ++f) T[f] += (double)t[f];
   }
 
+  if constexpr (NGB && PIPE) ngb_flush(pend, ngb_list, st.i);
   dsd = (double)dsmin;
   if constexpr (Q > 1) {
 #pragma unroll
@@ -355,7 +388,7 @@ __device__ __forceinline__ void wave_force(
   writer = ivalid && jsub == 0;
 }
 
-template <int ORDER, bool F64, int P, bool NGB>
+template <int ORDER, bool F64, int P, bool NGB, bool PIPE>
 __global__ __launch_bounds__(256) void k_force(
     const typename RecOf<ORDER, F64>::T* __restrict__ rec, const int2* __restrict__ idr, const int nj,
     const double* __restrict__ ipos, const double* __restrict__ ivel,
@@ -379,7 +412,7 @@ __global__ __launch_bounds__(256) void k_force(
   double dsd;
   int nnaddr, islot;
   bool writer;
-  wave_force<ORDER, F64, P, NGB>(rec, idr, ipos, ivel, iacc, ih2, iid, g * P, min(P, ni - g * P), jb, je, eps2d,
+  wave_force<ORDER, F64, P, NGB, PIPE>(rec, idr, ipos, ivel, iacc, ih2, iid, g * P, min(P, ni - g * P), jb, je, eps2d,
                                  ngb_cnt, ngb_list, T, dsd, nnaddr, islot, writer);
   if (writer) {
     const int i = g * P + islot;
@@ -705,14 +738,15 @@ static void launch_force_p(hipStream_t s, const ForceArgs& a, const ForceConfig&
   const long G = (a.ni + P - 1) / P;
   const int nb = blocks_for(G * c.S, 4);
   const Rec* rec = reinterpret_cast<const Rec*>(a.rec);
-  if (a.ngb_on)
-    hipLaunchKernelGGL((k_force<OR, FP, P, true>), dim3(nb), dim3(256), 0, s, rec, a.idr, a.nj, a.ipos, a.ivel,
-                       a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, a.part_addr, a.ngb_cnt,
-                       a.ngb_list, c.S, c.jchunk, c.smajor);
+#define SAPK_FORCE_ARGS rec, a.idr, a.nj, a.ipos, a.ivel, a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, \
+                        a.part_addr, a.ngb_cnt, a.ngb_list, c.S, c.jchunk, c.smajor
+  if (a.ngb_on && c.pipe)
+    hipLaunchKernelGGL((k_force<OR, FP, P, true, true>), dim3(nb), dim3(256), 0, s, SAPK_FORCE_ARGS);
+  else if (a.ngb_on)
+    hipLaunchKernelGGL((k_force<OR, FP, P, true, false>), dim3(nb), dim3(256), 0, s, SAPK_FORCE_ARGS);
   else
-    hipLaunchKernelGGL((k_force<OR, FP, P, false>), dim3(nb), dim3(256), 0, s, rec, a.idr, a.nj, a.ipos, a.ivel,
-                       a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, a.part_addr, a.ngb_cnt,
-                       a.ngb_list, c.S, c.jchunk, c.smajor);
+    hipLaunchKernelGGL((k_force<OR, FP, P, false, false>), dim3(nb), dim3(256), 0, s, SAPK_FORCE_ARGS);
+#undef SAPK_FORCE_ARGS
 }
 
 template <int OR, bool FP>

@@ -309,6 +309,11 @@ int Engine::set_j_particles(int n, const int* addr, const int* id, const double*
                             const double* mass, const double* a, const double* jrk, const double* v,
                             const double* x, const double* snp, const double* crk, const double* eps) {
   if (n < 0 || !addr || !id || !mass || !x) return fail("set_j_particles: bad arguments");
+  if (!open_) return fail("set_j_particles: library not open");
+  // the batch names all its addresses up front: grow the logical capacity once
+  int amax = -1;
+  for (int k = 0; k < n; ++k) amax = addr[k] > amax ? addr[k] : amax;
+  if (amax >= nj_max_) nj_max_ = amax + 1;
   for (int k = 0; k < n; ++k) {
     const size_t o = 3 * (size_t)k;
     const int rc = set_j_particle(addr[k], id[k], tj ? tj[k] : 0.0, dtj ? dtj[k] : 0.0, mass[k],

@@ -158,6 +158,11 @@ template <> __device__ __forceinline__ double rsq<double>(double x) {
   return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
+// Relative coordinate: FP64 subtraction, rounded to the working precision.
+// (The reference's DS kernel uses double-single pairs instead, kernels.cu:154-178;
+// on gfx950 the FP64 add and the convert issue at the FP32 rate, so the two
+// instructions per coordinate are cheaper than the three of the DS difference
+// and exact before the final rounding.)
 template <typename R> __device__ __forceinline__ R to_rel(double xj, double xi) { return (R)(xj - xi); }
 
 // Per-lane state of one i-particle.
@@ -180,7 +185,8 @@ __device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, c
   const R dy = to_rel<R>(r.y, s.y);
   const R dz = to_rel<R>(r.z, s.z);
   const R ds2 = fmaR<R>(dz, dz, fmaR<R>(dy, dy, dx * dx));
-  const bool use = CHECK ? (ok && (r.id != s.id)) : ok;
+  // bitwise & keeps the predicates branch-free (no exec-masked regions)
+  const bool use = CHECK ? (ok & (r.id != s.id)) : ok;
   const R inv0 = rsq<R>(ds2 + eps2);
   const R inv = use ? inv0 : R(0);
   const R invr2 = inv * inv;
@@ -225,11 +231,11 @@ __device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, c
     t[6] = t[6] + jz;
   }
   // nearest neighbour: strict <, first address wins inside a lane (kernels.cu:539-546)
-  const bool better = use && (ds2 < dsmin);
+  const bool better = use & (ds2 < dsmin);
   dsmin = better ? ds2 : dsmin;
   nnaddr = better ? j : nnaddr;
   // neighbour predicate h2_i > ds2 (kernels.cu:584); the caller appends
-  return NGB && use && (s.h2 > ds2);
+  return NGB & use & (s.h2 > ds2);
 }
 
 // Append id to the i-particle's list; rare (~32 per 1e5 pairs), so a global
@@ -257,7 +263,18 @@ __device__ __forceinline__ void ngb_flush(PendingNgb& p, int* __restrict__ ngb_l
   for (int u = 0; u < 4; ++u) p.m[u] = false;
 }
 
+// Empty register constraint: the values are computed before this point.
+template <int ORDER, typename R>
+__device__ __forceinline__ void settle(R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr) {
+#pragma unroll
+  for (int f = 0; f < nf<ORDER>(); ++f) asm volatile("" : "+v"(t[f]));
+  asm volatile("" : "+v"(dsmin), "+v"(nnaddr));
+}
+
 // Four consecutive j of one lane with a single neighbour branch for the group.
+// The group's arithmetic is settled before the branch; otherwise the compiler
+// sinks it below the (rarely taken) append block and keeps the four relative
+// positions live across it, which costs a wave of occupancy.
 template <int ORDER, bool NGB, bool CHECK, int Q, bool PIPE, typename R, typename Rec>
 __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const IState<ORDER, R>& s,
                                       const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
@@ -271,9 +288,13 @@ __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0,
     idj[u] = r.id;
     nb[u] = pair<ORDER, NGB, CHECK>(r, j, true, s, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
   }
+  if constexpr (NGB) settle<ORDER>(t, dsmin, nnaddr);
+  // the group test is a wave-uniform OR of the four compare masks (a bool OR
+  // reduction would be vectorised into per-lane bit arithmetic)
+  const bool any = NGB && (__ballot(nb[0]) | __ballot(nb[1]) | __ballot(nb[2]) | __ballot(nb[3])) != 0;
   if constexpr (NGB && PIPE) {
     ngb_flush(pend, ngb_list, s.i);
-    if (nb[0] | nb[1] | nb[2] | nb[3]) {
+    if (any) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (nb[u]) pend.k[u] = atomicAdd(&ngb_cnt[s.i], 1);
@@ -282,7 +303,7 @@ __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0,
       }
     }
   } else if constexpr (NGB) {
-    if (nb[0] | nb[1] | nb[2] | nb[3]) {
+    if (any) {
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         if (nb[u]) ngb_append(ngb_cnt, ngb_list, s.i, idj[u]);
@@ -334,10 +355,16 @@ __device__ __forceinline__ void wave_force(
   for (int u = 0; u < 4; ++u) { pend.m[u] = false; pend.k[u] = 0; pend.id[u] = 0; }
 
   const int nk = (je > jb) ? (je - jb + Q - 1) / Q : 0;  // iterations of the lane with jsub 0
-  for (int k0 = 0; k0 < nk; k0 += KT) {
-    R t[NF];
+  // FP32 pair terms are summed per fold of KT and folded into FP64 T; in FP64
+  // the pair terms go straight into t, which becomes T at the end
+  R t[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) t[f] = R(0);
+  for (int f = 0; f < NF; ++f) t[f] = R(0);
+  for (int k0 = 0; k0 < nk; k0 += KT) {
+    if constexpr (!F64) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) t[f] = R(0);
+    }
     const int kn = min(KT, nk - k0);
     const int jt = jb + k0 * Q + jsub;
     if (kn == KT && jb + (k0 + KT) * Q <= je) {
@@ -367,8 +394,14 @@ __device__ __forceinline__ void wave_force(
           ngb_append(ngb_cnt, ngb_list, st.i, r.id);
       }
     }
+    if constexpr (!F64) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) T[f] += (double)t[f];
+      for (int f = 0; f < NF; ++f) T[f] += (double)t[f];
+    }
+  }
+  if constexpr (F64) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) T[f] = (double)t[f];
   }
 
   if constexpr (NGB && PIPE) ngb_flush(pend, ngb_list, st.i);

@@ -37,6 +37,21 @@ import sapporo2_amd as sa  # noqa: E402
 from sapporo2_amd import ics  # noqa: E402
 
 METRIC = "Ginteractions/s Hermite-4 N=128K, mixed vs FP64 GPU at rel. force err ≤1e-6"
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "force_h4_mixed_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the mixed P = 64 force kernel over one step, as measured
+    by rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a known-size copy) and recorded
+    in the committed profile summary; None when that summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 FLOP_PER_INTERACTION = 60.0          # test_gravity_block.cpp:271
 PEAK_TFLOPS = {sa.DOUBLESINGLE: 157.3, sa.DOUBLE: 78.6}   # MI355X FP32 / FP64 vector
 EPS2 = 1e-6
@@ -394,7 +409,9 @@ def main():
                    "kernel": main_name},
         "roofline": {"bound": "valu", "achieved": round(r["achieved_tflops"], 3),
                      "peak": r["peak"], "unit": "TFLOP/s",
-                     "frac": round(r["achieved_tflops"] / r["peak"], 4), "traffic": None,
+                     "frac": round(r["achieved_tflops"] / r["peak"], 4),
+                     "traffic": measured_traffic() if main_name == "mixed" else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
                      "kernel": "k_force<H4,%s,P=64>" % main_name,
                      "mean_launch_ms": round(r["force_mean_ms"], 4),
                      "launches": r["force_launches"],

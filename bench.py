@@ -429,8 +429,9 @@ def main():
                                  "mean_launch_ms": round(d["force_mean_ms"], 4)}
         line["fp64_api_value"] = round(d["api_value"], 3)
     # predictor algorithmic bytes per j: reads t, x, v, a, j, m (14 doubles) + id; writes the
-    # working record (48 B mixed, 64 B FP64) + FP64 predicted velocity (mixed only, 24 B)
-    pb = 116 + (48 + 24 if main_name == "mixed" else 64)
+    # working record (48 B mixed, 64 B FP64) + FP64 predicted position and velocity (mixed
+    # only, 48 B, kept for read-back)
+    pb = 116 + (48 + 48 if main_name == "mixed" else 64)
     line["predictor"] = {"mean_launch_ms": round(r["pred_mean_ms"], 4), "launches": r["pred_launches"],
                          "bytes_per_j": pb, "n": w.n,
                          "achieved_GBps": round(pb * w.n / (r["pred_mean_ms"] * 1e-3) / 1e9, 1)

@@ -20,8 +20,11 @@
  *   predictor : the reference expression evaluated in IEEE double with no
  *               contraction (build with -ffp-contract=off).
  *   FP64 ds2  : dx = xj - xi (double); ds2 = fma(dz,dz, fma(dy,dy, dx*dx)).
- *   mixed ds2 : dx = (float)(xj - xi); ds2 = fmaf(dz,dz, fmaf(dy,dy, dx*dx)),
- *               h2 rounded to float.
+ *   mixed ds2 : positions as double-single pairs, hi = (float)x,
+ *               lo = (float)(x - hi) (kernels.cu:104-109, 163-167);
+ *               dx = (xj.hi - xi.hi) + (xj.lo - xi.lo) in float (kernels.cu:170-178);
+ *               ds2 = fmaf(dz,dz, fmaf(dy,dy, dx*dx)) (kernels.cu:567-570 with
+ *               multiply-add contraction); h2 rounded to float.
  *   pair kept : id_i != id_j                         (kernels.cu:565)
  *   neighbour : h2_i > ds2 (strict)                  (kernels.cu:584)
  *   NN        : smallest ds2 (strict <) over kept pairs; exact ties go to the
@@ -189,6 +192,20 @@ void orc_force(int order, int ni, const double *xi, const double *vi,
     }
 }
 
+/* Double-single split of a double (reference DSX::operator=(double),
+ * kernels.cu:163-167): hi is x rounded to float, lo the rounded remainder. */
+static inline void ds_split(double x, float *hi, float *lo)
+{
+    *hi = (float)x;
+    *lo = (float)(x - (double)*hi);
+}
+
+/* Relative coordinate of the DS kernel (DSX::operator-, kernels.cu:170-178). */
+static inline float ds_sub(float jh, float jl, float ih, float il)
+{
+    return (jh - ih) + (jl - il);
+}
+
 /* Nearest neighbour and neighbour lists with the mixed-precision (FP32)
  * distance contract of the header: the reference DS kernel keeps ds2 and
  * h2 in float (kernels.cu:419, 567-589).  nn_ds2 is the float ds2. */
@@ -202,16 +219,19 @@ void orc_neighbours_mixed(int ni, const double *xi, const double *h2i,
 #endif
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < ni; ++i) {
-        const double px = xi[3 * i], py = xi[3 * i + 1], pz = xi[3 * i + 2];
+        float ih[3], il[3];
+        for (int d = 0; d < 3; ++d) ds_split(xi[3 * i + d], &ih[d], &il[d]);
         const float h2 = h2i ? (float)h2i[i] : -1.0f;
         float dsmin = INFINITY;
         int nn = -1, cnt = 0, stored = 0;
         int *lst = (ngb_list && cap > 0) ? ngb_list + (size_t)i * cap : NULL;
         for (int j = 0; j < nj; ++j) {
             if (idi[i] == idj[j]) continue;
-            const float dx = (float)(xj[3 * j] - px);
-            const float dy = (float)(xj[3 * j + 1] - py);
-            const float dz = (float)(xj[3 * j + 2] - pz);
+            float jh[3], jl[3];
+            for (int d = 0; d < 3; ++d) ds_split(xj[3 * j + d], &jh[d], &jl[d]);
+            const float dx = ds_sub(jh[0], jl[0], ih[0], il[0]);
+            const float dy = ds_sub(jh[1], jl[1], ih[1], il[1]);
+            const float dz = ds_sub(jh[2], jl[2], ih[2], il[2]);
             const float ds2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
             if (ds2 < dsmin) { dsmin = ds2; nn = j; }
             if (h2 > ds2) {

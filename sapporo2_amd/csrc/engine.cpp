@@ -130,7 +130,7 @@ int Engine::close() {
   if (!open_) return 0;
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
-  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
+  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.ppos); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
   st_.cap = 0;
   for (auto& s : stage_) {
     hfree(s.d); hfree(s.ai); s.cap = 0; s.busy = false;
@@ -183,12 +183,14 @@ int Engine::ensure_store(int need) {
   // zeroed slots carry id 0, so every new tile's range starts at [0, 0]
   HIPCHK(hipMemsetAsync(ns.idr, 0, sizeof(int2) * (size_t)newcap / JTILE, stream_), "memset");
   if (!f64_) {
+    HIPCHK(dmalloc(&ns.ppos, (size_t)3 * newcap), "j-store ppos");
     HIPCHK(dmalloc(&ns.pvel, (size_t)3 * newcap), "j-store pvel");
     if (order_ >= ORDER_H6) HIPCHK(dmalloc(&ns.pacc, (size_t)3 * newcap), "j-store pacc");
   }
   HIPCHK(hipMemsetAsync(ns.raw, 0, sizeof(double) * nfr * (size_t)newcap, stream_), "memset");
   HIPCHK(hipMemsetAsync(ns.id, 0, sizeof(int) * (size_t)newcap, stream_), "memset");
   HIPCHK(hipMemsetAsync(ns.rec, 0, rb * newcap, stream_), "memset");
+  if (ns.ppos) HIPCHK(hipMemsetAsync(ns.ppos, 0, sizeof(double) * 3 * (size_t)newcap, stream_), "memset");
   if (ns.pvel) HIPCHK(hipMemsetAsync(ns.pvel, 0, sizeof(double) * 3 * (size_t)newcap, stream_), "memset");
   if (ns.pacc) HIPCHK(hipMemsetAsync(ns.pacc, 0, sizeof(double) * 3 * (size_t)newcap, stream_), "memset");
   if (oldcap > 0) {
@@ -200,6 +202,9 @@ int Engine::ensure_store(int need) {
     HIPCHK(hipMemcpyAsync(ns.idr, st_.idr, sizeof(int2) * (size_t)oldcap / JTILE, hipMemcpyDeviceToDevice,
                           stream_), "store copy");
     for (int d = 0; d < 3; ++d) {
+      if (ns.ppos)
+        HIPCHK(hipMemcpyAsync(ns.ppos + (size_t)d * newcap, st_.ppos + (size_t)d * oldcap,
+                              sizeof(double) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
       if (ns.pvel)
         HIPCHK(hipMemcpyAsync(ns.pvel + (size_t)d * newcap, st_.pvel + (size_t)d * oldcap,
                               sizeof(double) * oldcap, hipMemcpyDeviceToDevice, stream_), "store copy");
@@ -209,7 +214,7 @@ int Engine::ensure_store(int need) {
     }
     HIPCHK(hipStreamSynchronize(stream_), "store grow sync");
   }
-  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
+  dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.ppos); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
   st_ = ns;
   return 0;
 }

@@ -32,11 +32,16 @@ enum RawField {
 // Number of raw fields an update record carries for an order.
 constexpr int raw_fields(int order) { return order >= ORDER_H6 ? RF_COUNT : RF_SX; }
 
+// Double-single coordinate: hi = x rounded to float, lo = (x - hi) rounded
+// (the reference's DSX, kernels.cu:154-178).
+struct DSF { float hi, lo; };
+
 // Predicted j-records read by the force kernels (one array of structs, read
-// wave-uniformly through the scalar cache or per lane).  Positions stay in
-// FP64 in every variant: relative positions are formed by FP64 subtraction.
-struct JRecM4 { double x, y, z; float m, vx, vy, vz; int id, pad; };                  // 48 B
-struct JRecM6 { double x, y, z; float m, vx, vy, vz, ax, ay, az; int id, pad[2]; };   // 64 B
+// wave-uniformly through the scalar cache or per lane).  Mixed precision keeps
+// positions as double-single pairs; relative positions are
+// (xj.hi - xi.hi) + (xj.lo - xi.lo) in FP32.  FP64 keeps FP64 positions.
+struct JRecM4 { DSF x, y, z; float m, vx, vy, vz; int id, pad; };                  // 48 B
+struct JRecM6 { DSF x, y, z; float m, vx, vy, vz, ax, ay, az; int id, pad[2]; };   // 64 B
 struct JRecD4 { double x, y, z, m, vx, vy, vz; int id, pad; };                        // 64 B
 struct JRecD6 { double x, y, z, m, vx, vy, vz, ax, ay, az; int id, pad[3]; };         // 96 B
 
@@ -61,6 +66,7 @@ struct StoreView {
   double* raw;       // RF_COUNT (or raw_fields) x cap doubles
   int* id;           // cap ints
   void* rec;         // cap predicted records
+  double* ppos;      // mixed precision only: 3 x cap FP64 predicted position
   double* pvel;      // mixed precision only: 3 x cap FP64 predicted velocity
   double* pacc;      // mixed precision SIXTH only: 3 x cap FP64 predicted acc
   int2* idr;         // cap / JTILE tile id ranges (x = min, y = max)

@@ -11,9 +11,9 @@
 //                 i-particle over one contiguous j-range (the "j-split").  With
 //                 P = 64 every lane of a wave reads the same j, so the j-record
 //                 is fetched once per wave through the scalar cache into SGPRs
-//                 and feeds the VALU directly.  Mixed precision: relative
-//                 positions by FP64 subtraction rounded to float, pair terms in
-//                 FP32, per-tile FP32 sums folded into FP64 accumulators.
+//                 and feeds the VALU directly.  Mixed precision: double-single
+//                 positions, relative positions and pair terms in FP32,
+//                 per-tile FP32 sums folded into FP64 accumulators.
 //   k_finalize    deterministic fixed-order reduction of the S partial sums,
 //                 nearest-neighbour selection and neighbour-list sort.
 //   k_pack_*      compacts the per-i neighbour lists for read-back.
@@ -38,6 +38,16 @@ static_assert(sizeof(JRecM6) == 64, "record layout");
 static_assert(sizeof(JRecD4) == 64, "record layout");
 static_assert(sizeof(JRecD6) == 96, "record layout");
 
+// Double-single split (reference DSX::operator=(double), kernels.cu:163-167).
+__device__ __forceinline__ DSF to_ds(double x) {
+  DSF d;
+  d.hi = (float)x;
+  d.lo = (float)(x - (double)d.hi);
+  return d;
+}
+__device__ __forceinline__ void put_pos(double& dst, double x) { dst = x; }
+__device__ __forceinline__ void put_pos(DSF& dst, double x) { dst = to_ds(x); }
+
 // Write the working-precision record of particle j from FP64 values.
 template <int ORDER, bool F64>
 __device__ __forceinline__ void put_rec(void* base, int j, double x, double y, double z, double m,
@@ -46,7 +56,7 @@ __device__ __forceinline__ void put_rec(void* base, int j, double x, double y, d
   using Rec = typename RecOf<ORDER, F64>::T;
   using R = typename std::conditional<F64, double, float>::type;
   Rec r{};
-  r.x = x; r.y = y; r.z = z;
+  put_pos(r.x, x); put_pos(r.y, y); put_pos(r.z, z);
   r.m = (R)m; r.vx = (R)vx; r.vy = (R)vy; r.vz = (R)vz;
   if constexpr (ORDER >= ORDER_H6) { r.ax = (R)ax; r.ay = (R)ay; r.az = (R)az; }
   r.id = id;
@@ -77,6 +87,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     put_rec<ORDER, F64>(a.st.rec, addr, u[RF_X], u[RF_Y], u[RF_Z], u[RF_M], u[RF_VX], u[RF_VY],
                         u[RF_VZ], ax, ay, az, id);
     if constexpr (!F64) {
+      a.st.ppos[addr] = u[RF_X]; a.st.ppos[cap + addr] = u[RF_Y]; a.st.ppos[2 * cap + addr] = u[RF_Z];
       a.st.pvel[addr] = u[RF_VX]; a.st.pvel[cap + addr] = u[RF_VY]; a.st.pvel[2 * cap + addr] = u[RF_VZ];
       if constexpr (ORDER >= ORDER_H6) {
         a.st.pacc[addr] = ax; a.st.pacc[cap + addr] = ay; a.st.pacc[2 * cap + addr] = az;
@@ -134,6 +145,7 @@ __global__ __launch_bounds__(256) void k_predict(PredictArgs a) {
   put_rec<ORDER, F64>(a.st.rec, j, p[0], p[1], p[2], a.st.raw[RF_M * cap + j], v[0], v[1], v[2],
                       ac[0], ac[1], ac[2], a.st.id[j]);
   if constexpr (!F64) {
+    a.st.ppos[j] = p[0]; a.st.ppos[cap + j] = p[1]; a.st.ppos[2 * cap + j] = p[2];
     a.st.pvel[j] = v[0]; a.st.pvel[cap + j] = v[1]; a.st.pvel[2 * cap + j] = v[2];
     if constexpr (ORDER >= ORDER_H6) {
       a.st.pacc[j] = ac[0]; a.st.pacc[cap + j] = ac[1]; a.st.pacc[2 * cap + j] = ac[2];
@@ -158,17 +170,20 @@ template <> __device__ __forceinline__ double rsq<double>(double x) {
   return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
 
-// Relative coordinate: FP64 subtraction, rounded to the working precision.
-// (The reference's DS kernel uses double-single pairs instead, kernels.cu:154-178;
-// on gfx950 the FP64 add and the convert issue at the FP32 rate, so the two
-// instructions per coordinate are cheaper than the three of the DS difference
-// and exact before the final rounding.)
-template <typename R> __device__ __forceinline__ R to_rel(double xj, double xi) { return (R)(xj - xi); }
+// Relative coordinate xj - xi: FP64 subtraction, or the double-single
+// difference (xj.hi - xi.hi) + (xj.lo - xi.lo) in FP32 (kernels.cu:170-178).
+__device__ __forceinline__ double to_rel(double xj, double xi) { return xj - xi; }
+__device__ __forceinline__ float to_rel(const DSF& xj, const DSF& xi) { return (xj.hi - xi.hi) + (xj.lo - xi.lo); }
+
+template <typename R> struct PosOf { using T = DSF; };
+template <> struct PosOf<double> { using T = double; };
+__device__ __forceinline__ void get_pos(double& dst, double x) { dst = x; }
+__device__ __forceinline__ void get_pos(DSF& dst, double x) { dst = to_ds(x); }
 
 // Per-lane state of one i-particle.
 template <int ORDER, typename R>
 struct IState {
-  double x, y, z;
+  typename PosOf<R>::T x, y, z;
   R vx, vy, vz, ax, ay, az, h2;
   int id, i;
 };
@@ -181,9 +196,9 @@ template <int ORDER, bool NGB, bool CHECK, typename R, typename Rec>
 __device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, const IState<ORDER, R>& s,
                                      const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
                                      int* __restrict__ ngb_cnt, int* __restrict__ ngb_list) {
-  const R dx = to_rel<R>(r.x, s.x);
-  const R dy = to_rel<R>(r.y, s.y);
-  const R dz = to_rel<R>(r.z, s.z);
+  const R dx = to_rel(r.x, s.x);
+  const R dy = to_rel(r.y, s.y);
+  const R dz = to_rel(r.z, s.z);
   const R ds2 = fmaR<R>(dz, dz, fmaR<R>(dy, dy, dx * dx));
   // bitwise & keeps the predicates branch-free (no exec-masked regions)
   const bool use = CHECK ? (ok & (r.id != s.id)) : ok;
@@ -335,7 +350,7 @@ __device__ __forceinline__ void wave_force(
   const int il = i0 + (ivalid ? islot : 0);
 
   IState<ORDER, R> st;
-  st.x = ipos[3 * il]; st.y = ipos[3 * il + 1]; st.z = ipos[3 * il + 2];
+  get_pos(st.x, ipos[3 * il]); get_pos(st.y, ipos[3 * il + 1]); get_pos(st.z, ipos[3 * il + 2]);
   st.vx = (R)ivel[3 * il]; st.vy = (R)ivel[3 * il + 1]; st.vz = (R)ivel[3 * il + 2];
   st.ax = st.ay = st.az = R(0);
   if constexpr (ORDER >= ORDER_H6) {
@@ -711,13 +726,14 @@ __global__ __launch_bounds__(256) void k_gather_pred(GatherArgs a) {
   const Rec r = reinterpret_cast<const Rec*>(a.st.rec)[j];
   const size_t cap = (size_t)a.st.cap;
   double* o = a.out + (size_t)j * 10;
-  o[0] = r.x; o[1] = r.y; o[2] = r.z;
   o[3] = a.st.raw[RF_M * cap + j];
   if constexpr (F64) {
+    o[0] = r.x; o[1] = r.y; o[2] = r.z;
     o[4] = r.vx; o[5] = r.vy; o[6] = r.vz;
     if constexpr (ORDER >= ORDER_H6) { o[7] = r.ax; o[8] = r.ay; o[9] = r.az; }
     else { o[7] = o[8] = o[9] = 0.0; }
   } else {
+    o[0] = a.st.ppos[j]; o[1] = a.st.ppos[cap + j]; o[2] = a.st.ppos[2 * cap + j];
     o[4] = a.st.pvel[j]; o[5] = a.st.pvel[cap + j]; o[6] = a.st.pvel[2 * cap + j];
     if constexpr (ORDER >= ORDER_H6) { o[7] = a.st.pacc[j]; o[8] = a.st.pacc[cap + j]; o[9] = a.st.pacc[2 * cap + j]; }
     else { o[7] = o[8] = o[9] = 0.0; }

@@ -66,9 +66,35 @@ def fma_exact(a, b, c):
     return f64(Fraction(a) * Fraction(b) + Fraction(c))
 
 
+def ds_split_exact(x):
+    # hi = x rounded to float; lo = the remainder x - hi (exact in rationals), rounded
+    hi = f32(Fraction(x))
+    return hi, f32(Fraction(x) - Fraction(hi))
+
+
+def ds_sub_exact(xj, xi):
+    # (xj.hi - xi.hi) + (xj.lo - xi.lo), each operation rounded to float
+    jh, jl = ds_split_exact(xj)
+    ih, il = ds_split_exact(xi)
+    return f32(Fraction(f32(Fraction(jh) - Fraction(ih))) + Fraction(f32(Fraction(jl) - Fraction(il))))
+
+
 def ds2_mixed_exact(xj, xi):
-    d = [f32(f64(Fraction(xj[k]) - Fraction(xi[k]))) for k in range(3)]
+    d = [ds_sub_exact(xj[k], xi[k]) for k in range(3)]
     return fmaf_exact(d[2], d[2], fmaf_exact(d[1], d[1], f32(Fraction(d[0]) * Fraction(d[0]))))
+
+
+def test_ds_difference_bounds():
+    """The double-single difference is within 2 float ulps of the exact difference,
+    and equals the rounded exact difference when both inputs are floats."""
+    for _ in range(400):
+        xi = float(RNG.normal(0, 10.0))
+        xj = xi + float(RNG.normal(0, 10.0 ** RNG.uniform(-6, 1)))
+        d = ds_sub_exact(xj, xi)
+        exact = Fraction(xj) - Fraction(xi)
+        assert abs(Fraction(d) - exact) <= 2 * Fraction(float(np.spacing(np.float32(abs(float(exact)))))), (xj, xi)
+        fi, fj = float(np.float32(xi)), float(np.float32(xj))
+        assert ds_sub_exact(fj, fi) == f32(Fraction(fj) - Fraction(fi))
 
 
 def ds2_f64_exact(xj, xi):

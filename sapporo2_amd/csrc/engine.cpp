@@ -761,7 +761,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   fz.part = part_d_;
   fz.part_ds2 = pds2_d_;
   fz.part_addr = paddr_d_;
-  fz.S = cfg.S;
+  fz.S = partial_rows(cfg);
   fz.ni = ni;
   fz.rec = st_.rec;
   fz.ngb_cnt = cnt_d_;

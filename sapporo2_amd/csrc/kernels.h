@@ -114,6 +114,11 @@ struct ForceArgs {
 // smajor: wave-to-split mapping; pipe: neighbour-id stores deferred by one group.
 struct ForceConfig { int P; int S; int jchunk; int smajor; int pipe; };
 
+// Partial-sum rows a force launch writes per i-particle.  With S a multiple of
+// 4 and split-minor wave order the 4 waves of a work-group are 4 consecutive
+// splits of one i-group; they combine their sums in LDS and write one row.
+inline int partial_rows(const ForceConfig& c) { return (c.S % 4 == 0 && !c.smajor) ? c.S / 4 : c.S; }
+
 struct FinalizeArgs {
   const double* part; const double* part_ds2; const int* part_addr;
   int S, ni;

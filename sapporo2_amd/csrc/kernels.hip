@@ -443,7 +443,8 @@ __global__ __launch_bounds__(256) void k_force(
     const double* __restrict__ iacc, const double* __restrict__ ih2,
     const int* __restrict__ iid, const int ni, const double eps2d,
     double* __restrict__ part, double* __restrict__ part_ds2, int* __restrict__ part_addr,
-    int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, const int S, const int jchunk, const int smajor) {
+    int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, const int S, const int jchunk, const int smajor,
+    const int blockred) {
   constexpr int NF = nf<ORDER>();
   // wave-uniform by construction; readfirstlane lets the compiler keep the
   // j-loop bookkeeping and (for P = 64) the j-record loads in SGPRs
@@ -462,6 +463,50 @@ __global__ __launch_bounds__(256) void k_force(
   bool writer;
   wave_force<ORDER, F64, P, NGB, PIPE>(rec, idr, ipos, ivel, iacc, ih2, iid, g * P, min(P, ni - g * P), jb, je, eps2d,
                                  ngb_cnt, ngb_list, T, dsd, nnaddr, islot, writer);
+  if (blockred) {
+    // The block's waves are splits 4q..4q+3 of one i-group (S % 4 == 0, so
+    // every block is whole and no wave left early).  Their sums go through
+    // LDS four fields at a time; wave w adds field c0 + w over the four
+    // splits in the fixed order (s0 + s1) + (s2 + s3) and writes one row.
+    // Wave 3 also takes the nearest neighbour.  The barriers bound what any
+    // wave holds at once, so the epilogue does not raise the register count.
+    __shared__ double red_t[4][4][64];
+    __shared__ double red_d[4][64];
+    __shared__ int red_a[4][64];
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int i = g * P + islot;
+    const int S4 = S >> 2, row = sidx >> 2;
+    red_d[w][lane] = dsd;
+    red_a[w][lane] = nnaddr;
+#pragma unroll
+    for (int c0 = 0; c0 < NF; c0 += 4) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (c0 + k < NF) red_t[w][k][lane] = T[c0 + k];
+      __syncthreads();
+      if (c0 + w < NF && writer) {
+        const double v = (red_t[0][w][lane] + red_t[1][w][lane]) + (red_t[2][w][lane] + red_t[3][w][lane]);
+        part[((size_t)(c0 + w) * S4 + row) * ni + i] = v;
+      }
+      if (c0 == 0 && w == 3 && writer) {
+        double bd = red_d[0][lane];
+        int ba = red_a[0][lane];
+#pragma unroll
+        for (int u = 1; u < 4; ++u) {
+          const double od = red_d[u][lane];
+          const int oa = red_a[u][lane];
+          const bool take = (od < bd) || (od == bd && oa < ba);
+          bd = take ? od : bd;
+          ba = take ? oa : ba;
+        }
+        part_ds2[(size_t)row * ni + i] = bd;
+        part_addr[(size_t)row * ni + i] = ba;
+      }
+      __syncthreads();
+    }
+    return;
+  }
   if (writer) {
     const int i = g * P + islot;
 #pragma unroll
@@ -788,7 +833,7 @@ static void launch_force_p(hipStream_t s, const ForceArgs& a, const ForceConfig&
   const int nb = blocks_for(G * c.S, 4);
   const Rec* rec = reinterpret_cast<const Rec*>(a.rec);
 #define SAPK_FORCE_ARGS rec, a.idr, a.nj, a.ipos, a.ivel, a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, \
-                        a.part_addr, a.ngb_cnt, a.ngb_list, c.S, c.jchunk, c.smajor
+                        a.part_addr, a.ngb_cnt, a.ngb_list, c.S, c.jchunk, c.smajor, partial_rows(c) != c.S ? 1 : 0
   if (a.ngb_on && c.pipe)
     hipLaunchKernelGGL((k_force<OR, FP, P, true, true>), dim3(nb), dim3(256), 0, s, SAPK_FORCE_ARGS);
   else if (a.ngb_on)

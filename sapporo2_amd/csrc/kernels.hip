@@ -291,9 +291,10 @@ __device__ __forceinline__ void settle(R (&t)[nf<ORDER>()], R& dsmin, int& nnadd
 // sinks it below the (rarely taken) append block and keeps the four relative
 // positions live across it, which costs a wave of occupancy.
 template <int ORDER, bool NGB, bool CHECK, int Q, bool PIPE, typename R, typename Rec>
-__device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const IState<ORDER, R>& s,
-                                      const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
-                                      int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, PendingNgb& pend) {
+__device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const int k0,
+                                      const IState<ORDER, R>& s, const R eps2, R (&t)[nf<ORDER>()], R& dsmin,
+                                      int& nnk, int* __restrict__ ngb_cnt, int* __restrict__ ngb_list,
+                                      PendingNgb& pend) {
   bool nb[4];
   int idj[4];
 #pragma unroll
@@ -301,9 +302,9 @@ __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0,
     const int j = j0 + u * Q;
     const Rec& r = rec[j];
     idj[u] = r.id;
-    nb[u] = pair<ORDER, NGB, CHECK>(r, j, true, s, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list);
+    nb[u] = pair<ORDER, NGB, CHECK>(r, k0 + u, true, s, eps2, t, dsmin, nnk, ngb_cnt, ngb_list);
   }
-  if constexpr (NGB) settle<ORDER>(t, dsmin, nnaddr);
+  if constexpr (NGB) settle<ORDER>(t, dsmin, nnk);
   // the group test is a wave-uniform OR of the four compare masks (a bool OR
   // reduction would be vectorised into per-lane bit arithmetic)
   const bool any = NGB && (__ballot(nb[0]) | __ballot(nb[1]) | __ballot(nb[2]) | __ballot(nb[3])) != 0;
@@ -392,12 +393,27 @@ __device__ __forceinline__ void wave_force(
           check = __ballot(st.id >= r.x && st.id <= r.y) != 0;
         }
       }
+      // The nearest-neighbour candidate is recorded as its index within a
+      // sub-block of 16 pairs (a compile-time inline constant: no register
+      // copy of the j index per pair) and turned into an address per sub-block.
       if (check) {
-        for (int k = 0; k < KT; k += 4)
-          pair4<ORDER, NGB, true, Q, PIPE>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list, pend);
+        for (int k = 0; k < KT; k += 16) {
+          int nnk = -1;
+#pragma unroll
+          for (int q = 0; q < 16; q += 4)
+            pair4<ORDER, NGB, true, Q, PIPE>(rec, jt + (k + q) * Q, q, st, eps2, t, dsmin, nnk, ngb_cnt, ngb_list,
+                                             pend);
+          nnaddr = nnk >= 0 ? jt + (k + nnk) * Q : nnaddr;
+        }
       } else {
-        for (int k = 0; k < KT; k += 4)
-          pair4<ORDER, NGB, false, Q, PIPE>(rec, jt + k * Q, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list, pend);
+        for (int k = 0; k < KT; k += 16) {
+          int nnk = -1;
+#pragma unroll
+          for (int q = 0; q < 16; q += 4)
+            pair4<ORDER, NGB, false, Q, PIPE>(rec, jt + (k + q) * Q, q, st, eps2, t, dsmin, nnk, ngb_cnt, ngb_list,
+                                              pend);
+          nnaddr = nnk >= 0 ? jt + (k + nnk) * Q : nnaddr;
+        }
       }
     } else {
       if constexpr (NGB && PIPE) ngb_flush(pend, ngb_list, st.i);

@@ -97,6 +97,14 @@ int sapamd_set_split_major(sapamd_engine *e, int on);
  * atomic's latency overlaps arithmetic (more registers); off: immediate */
 int sapamd_set_ngb_pipeline(sapamd_engine *e, int on);
 int sapamd_set_timing(sapamd_engine *e, int on);
+/* on: each force call's device work (scatter, predictor, force, finalize) is
+ * captured as a hipGraph once per launch shape and replayed; the per-call
+ * values travel in a device-side descriptor.  Results are identical to the
+ * direct launches; per-kernel timings (last_force_ms, last_predict_ms) are 0.
+ * No reference counterpart (the reference launches each kernel, sapporohostclass.cpp:1442-1602).
+ * Default: off, or $SAPPORO2_AMD_GRAPHS. */
+int sapamd_set_graphs(sapamd_engine *e, int on);
+int sapamd_graph_count(sapamd_engine *e);
 float sapamd_last_force_ms(sapamd_engine *e);
 float sapamd_last_predict_ms(sapamd_engine *e);
 /* device time of the last force call from the first kernel after its input

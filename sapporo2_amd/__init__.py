@@ -51,7 +51,7 @@ ENGINE_SYMBOLS = [
     "sapamd_set_time", "sapamd_set_no_time", "sapamd_set_j_particle", "sapamd_start_grav_calc",
     "sapamd_get_grav_results", "sapamd_set_j_particles", "sapamd_last_device_ms", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
-    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing",
+    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing", "sapamd_set_graphs", "sapamd_graph_count",
     "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
     "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
     "sapamd_start_grav_calc_clusters",
@@ -95,6 +95,8 @@ def lib():
         L.sapamd_retrieve_j_particle_state.argtypes = [_vp, ctypes.c_int] + [_dp] * 10
         L.sapamd_set_force_override.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.sapamd_set_timing.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_set_graphs.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_graph_count.argtypes = [_vp]
         L.sapamd_set_split_major.argtypes = [_vp, ctypes.c_int]
         L.sapamd_set_ngb_pipeline.argtypes = [_vp, ctypes.c_int]
         L.sapamd_last_force_ms.argtypes = [_vp]
@@ -305,6 +307,13 @@ class Engine:
 
     def set_timing(self, on=True):
         self._L.sapamd_set_timing(self._h, 1 if on else 0)
+
+    def set_graphs(self, on=True):
+        """Capture each call's device work as a hipGraph per launch shape and replay it."""
+        self._L.sapamd_set_graphs(self._h, 1 if on else 0)
+
+    def graph_count(self):
+        return int(self._L.sapamd_graph_count(self._h))
 
     def last_force_ms(self):
         return float(self._L.sapamd_last_force_ms(self._h))

@@ -100,6 +100,17 @@ int sapamd_set_timing(sapamd_engine* e, int on) {
   e->eng.set_timing(on != 0);
   return 0;
 }
+
+int sapamd_set_graphs(sapamd_engine* e, int on) {
+  if (!e) return -1;
+  e->eng.set_graphs(on != 0);
+  return 0;
+}
+
+int sapamd_graph_count(sapamd_engine* e) {
+  if (!e) return -1;
+  return e->eng.graph_count();
+}
 float sapamd_last_force_ms(sapamd_engine* e) { return e ? e->eng.last_force_ms() : 0.f; }
 float sapamd_last_predict_ms(sapamd_engine* e) { return e ? e->eng.last_predict_ms() : 0.f; }
 float sapamd_last_device_ms(sapamd_engine* e) { return e ? e->eng.last_device_ms() : 0.f; }

@@ -107,6 +107,11 @@ int Engine::open(int device, int order, int precision) {
   HIPCHK(hipEventCreate(&ev_p0_), "hipEventCreate");
   HIPCHK(hipEventCreate(&ev_p1_), "hipEventCreate");
   for (auto& s : stage_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
+  HIPCHK(hipEventCreateWithFlags(&ev_desc_, hipEventDisableTiming), "hipEventCreate");
+  HIPCHK(hmalloc(&desc_h_, 1), "pinned call descriptor");
+  HIPCHK(dmalloc(&desc_d_, 1), "call descriptor");
+  desc_pending_ = false;
+  if (const char* g = getenv("SAPPORO2_AMD_GRAPHS")) graphs_ = atoi(g) != 0;
 
   icap_ = 0;
   if (ensure_icap(NPIPES) != 0) return -1;
@@ -130,6 +135,11 @@ int Engine::close() {
   if (!open_) return 0;
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
+  drop_graphs();
+  hfree(desc_h_); dfree(desc_d_);
+  if (ev_desc_) (void)hipEventDestroy(ev_desc_);
+  ev_desc_ = nullptr;
+  desc_pending_ = false;
   dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.ppos); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
   st_.cap = 0;
   for (auto& s : stage_) {
@@ -214,6 +224,7 @@ int Engine::ensure_store(int need) {
     }
     HIPCHK(hipStreamSynchronize(stream_), "store grow sync");
   }
+  ++gen_;
   dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.ppos); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
   st_ = ns;
   return 0;
@@ -346,6 +357,7 @@ int Engine::upload_updates(int* nupd) {
     HIPCHK(dmalloc(&upd_d_, (size_t)nc * nf), "device staging");
     HIPCHK(dmalloc(&upd_ai_d_, (size_t)nc * 2), "device staging ids");
     upd_cap_d_ = nc;
+    ++gen_;
   }
   Stage& sg = stage_[cur_];
   HIPCHK(hipMemcpyAsync(upd_d_, sg.d, sizeof(double) * (size_t)n * nf, hipMemcpyHostToDevice, stream_),
@@ -360,23 +372,50 @@ int Engine::upload_updates(int* nupd) {
   return 0;
 }
 
+// Work-groups of the scatter launch: grid-stride over the descriptor's counts,
+// rounded up to a power of two so that calls of similar size share a shape.
+int Engine::scatter_blocks(int n, int ni_reset) const {
+  const long m = std::max(n, ni_reset);
+  if (m <= 0) return 0;
+  int nb = 1;
+  while ((long)nb * 256 < m && nb < 4096) nb <<= 1;
+  return nb;
+}
+
 int Engine::scatter_updates(int n, int ni_reset) {
-  if (n <= 0 && ni_reset <= 0) return 0;
   ScatterArgs a{};
   a.st = st_;
   a.upd = upd_d_;
   a.upd_ai = upd_ai_d_;
-  a.nupd = n;
   a.nf = raw_fields(order_);
   a.ngb_cnt = cnt_d_;
-  a.ni_reset = ni_reset;
+  a.cd = desc_d_;
+  a.grid_blocks = scatter_blocks(n, ni_reset);
+  if (a.grid_blocks == 0) return 0;
   if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
+  return 0;
+}
+
+// Writes the call descriptor (pinned host copy) and queues its upload.  The
+// previous upload has to have read the host copy before it is rewritten.
+int Engine::post_desc(int ni, int nupd, int ni_reset, double eps2) {
+  if (desc_pending_) HIPCHK(hipEventSynchronize(ev_desc_), "descriptor wait");
+  desc_h_->ti = t_i_;
+  desc_h_->eps2 = eps2;
+  desc_h_->ni = ni;
+  desc_h_->nupd = nupd;
+  desc_h_->ni_reset = ni_reset;
+  desc_h_->pad = 0;
+  HIPCHK(hipMemcpyAsync(desc_d_, desc_h_, sizeof(CallDesc), hipMemcpyHostToDevice, stream_), "descriptor upload");
+  HIPCHK(hipEventRecord(ev_desc_, stream_), "event");
+  desc_pending_ = true;
   return 0;
 }
 
 int Engine::flush_updates(int ni_reset) {
   int n = 0;
   if (upload_updates(&n) != 0) return -1;
+  if (post_desc(0, n, ni_reset, 0.0) != 0) return -1;
   return scatter_updates(n, ni_reset);
 }
 
@@ -387,11 +426,12 @@ int Engine::predict_range(int j0, int j1) {
   p.st = st_;
   p.j0 = j0;
   p.j1 = j1;
-  p.ti = t_i_;
-  if (timing_) HIPCHK(hipEventRecord(ev_p0_, stream_), "event");
+  p.cd = desc_d_;
+  const bool ev = timing_ && !capturing_;
+  if (ev) HIPCHK(hipEventRecord(ev_p0_, stream_), "event");
   if (launch_predict(stream_, order_, f64_, p) != 0) return fail("predict launch", hipGetLastError());
-  if (timing_) HIPCHK(hipEventRecord(ev_p1_, stream_), "event");
-  timed_predict_ = timing_;
+  if (ev) HIPCHK(hipEventRecord(ev_p1_, stream_), "event");
+  timed_predict_ = ev;
   return 0;
 }
 
@@ -405,6 +445,7 @@ int Engine::ensure_partials(size_t entries) {
   HIPCHK(dmalloc(&pds2_d_, nc), "partials");
   HIPCHK(dmalloc(&paddr_d_, nc), "partials");
   part_cap_ = nc;
+  ++gen_;
   return 0;
 }
 
@@ -510,6 +551,7 @@ int Engine::ensure_icap(int need) {
   HIPCHK(hipMemsetAsync(cnt_d_, 0, nI * sizeof(int), stream_), "memset");
   icap_ = need;
   have_results_ = false;
+  ++gen_;
   return 0;
 }
 
@@ -632,6 +674,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
 
   int nupd = 0;
   if (upload_updates(&nupd) != 0) return -1;
+  if (post_desc(ni, nupd, ngb_on ? ni : 0, 0.0) != 0) return -1;
   if (timing_) HIPCHK(hipEventRecord(ev_d0_, stream_), "event");
   if (scatter_updates(nupd, ngb_on ? ni : 0) != 0) return -1;
   if (predict_) {
@@ -660,7 +703,8 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ea.ngb_list = list_d_;
   ea.ngb_on = ngb_on ? 1 : 0;
   FinalizeArgs fz{};
-  fz.ni = ni;
+  fz.cd = desc_d_;
+  fz.grid_ni = ni;
   fz.rec = st_.rec;
   fz.ngb_cnt = cnt_d_;
   fz.ngb_list = list_d_;
@@ -674,6 +718,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
     HIPCHK(hipEventRecord(ev_f1_, stream_), "event");
     HIPCHK(hipEventRecord(ev_d1_, stream_), "event");
   }
+  timed_force_ = timing_;
   HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
                         hipMemcpyDeviceToHost, stream_), "result download");
   HIPCHK(hipEventRecord(ev_done_, stream_), "event");
@@ -723,52 +768,28 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
 
   int nupd = 0;
   if (upload_updates(&nupd) != 0) return -1;
+  if (post_desc(ni, nupd, ngb_on ? ni : 0, eps2) != 0) return -1;
+  const bool predict = predict_;
+  predict_ = false;  // reference 698
+  const ForceConfig cfg = choose_config(ni, nj, ngb_on);
+  // With graphs the grid covers the power-of-two bucket of ni (calls of one
+  // bucket share a captured sequence); direct launches cover ni exactly.  The
+  // finalize shape follows the bucket in both cases, so results agree.
+  int bucket = 1;
+  while (bucket < ni) bucket <<= 1;
+  const int grid_ni = graphs_ ? std::min(bucket, NPIPES) : ni;
+  if (ensure_partials((size_t)cfg.S * grid_ni) != 0) return -1;
+  const int sblocks = scatter_blocks(nupd, ngb_on ? ni : 0);
   // device window: everything after the host->device copies
   if (timing_) HIPCHK(hipEventRecord(ev_d0_, stream_), "event");
-  if (scatter_updates(nupd, ngb_on ? ni : 0) != 0) return -1;
-  if (predict_) {
-    if (predict_range(0, nj) != 0) return -1;
+  if (graphs_) {
+    if (launch_call_graph(grid_ni, nj, cfg, ngb_on, predict, sblocks, bucket >= 1024) != 0) return -1;
+    timed_predict_ = timed_force_ = false;
   } else {
-    timed_predict_ = false;
+    if (enqueue_call(grid_ni, nj, cfg, ngb_on, predict, sblocks, bucket >= 1024) != 0) return -1;
+    if (!predict) timed_predict_ = false;
+    timed_force_ = timing_;
   }
-  predict_ = false;  // reference 698
-
-  const ForceConfig cfg = choose_config(ni, nj, ngb_on);
-  if (ensure_partials((size_t)cfg.S * ni) != 0) return -1;
-  ForceArgs fa{};
-  fa.rec = st_.rec;
-  fa.idr = st_.idr;
-  fa.nj = nj;
-  fa.ipos = ibuf_d_;
-  fa.ivel = ibuf_d_ + 3 * (size_t)ni;
-  fa.iacc = ibuf_d_ + 6 * (size_t)ni;
-  fa.ih2 = ibuf_d_ + (bh - bx);
-  fa.iid = iid_d_;
-  fa.ni = ni;
-  fa.eps2 = eps2;
-  fa.part = part_d_;
-  fa.part_ds2 = pds2_d_;
-  fa.part_addr = paddr_d_;
-  fa.ngb_cnt = cnt_d_;
-  fa.ngb_list = list_d_;
-  fa.ngb_on = ngb_on ? 1 : 0;
-  if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
-  const int rc = launch_force(stream_, order_, f64_, fa, cfg);
-  if (rc != 0) return fail("force launch", hipGetLastError());
-  if (timing_) HIPCHK(hipEventRecord(ev_f1_, stream_), "event");
-
-  FinalizeArgs fz{};
-  fz.part = part_d_;
-  fz.part_ds2 = pds2_d_;
-  fz.part_addr = paddr_d_;
-  fz.S = partial_rows(cfg);
-  fz.ni = ni;
-  fz.rec = st_.rec;
-  fz.ngb_cnt = cnt_d_;
-  fz.ngb_list = list_d_;
-  fz.out = out_d_;
-  fz.ngb_on = fa.ngb_on;
-  if (launch_finalize(stream_, order_, f64_, fz) != 0) return fail("finalize launch", hipGetLastError());
   if (timing_) HIPCHK(hipEventRecord(ev_d1_, stream_), "event");
   HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
                         hipMemcpyDeviceToHost, stream_), "result download");
@@ -784,6 +805,120 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   return 0;
 }
 
+// The device work of one force call: [scatter] -> [predict] -> force ->
+// finalize.  Every per-call value comes from the descriptor, so the sequence
+// depends only on its arguments here (the key of a captured graph).
+int Engine::enqueue_call(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_on, bool predict, int sblocks,
+                         bool rows) {
+  if (sblocks > 0) {
+    ScatterArgs a{};
+    a.st = st_;
+    a.upd = upd_d_;
+    a.upd_ai = upd_ai_d_;
+    a.nf = raw_fields(order_);
+    a.ngb_cnt = cnt_d_;
+    a.cd = desc_d_;
+    a.grid_blocks = sblocks;
+    if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
+  }
+  if (predict && predict_range(0, nj) != 0) return -1;
+  ForceArgs fa{};
+  fa.rec = st_.rec;
+  fa.idr = st_.idr;
+  fa.nj = nj;
+  fa.ibuf = ibuf_d_;
+  fa.iid = iid_d_;
+  fa.cd = desc_d_;
+  fa.grid_ni = grid_ni;
+  fa.part = part_d_;
+  fa.part_ds2 = pds2_d_;
+  fa.part_addr = paddr_d_;
+  fa.ngb_cnt = cnt_d_;
+  fa.ngb_list = list_d_;
+  fa.ngb_on = ngb_on ? 1 : 0;
+  const bool ev = timing_ && !capturing_;
+  if (ev) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
+  if (launch_force(stream_, order_, f64_, fa, cfg) != 0) return fail("force launch", hipGetLastError());
+  if (ev) HIPCHK(hipEventRecord(ev_f1_, stream_), "event");
+  FinalizeArgs fz{};
+  fz.part = part_d_;
+  fz.part_ds2 = pds2_d_;
+  fz.part_addr = paddr_d_;
+  fz.S = partial_rows(cfg);
+  fz.cd = desc_d_;
+  fz.grid_ni = grid_ni;
+  fz.rows = rows ? 1 : 0;
+  fz.rec = st_.rec;
+  fz.ngb_cnt = cnt_d_;
+  fz.ngb_list = list_d_;
+  fz.out = out_d_;
+  fz.ngb_on = fa.ngb_on;
+  if (launch_finalize(stream_, order_, f64_, fz) != 0) return fail("finalize launch", hipGetLastError());
+  return 0;
+}
+
+constexpr size_t MAX_GRAPHS = 64;
+
+// Replays the captured sequence of this launch shape, capturing it first if
+// needed (least recently used graphs are dropped beyond MAX_GRAPHS; graphs
+// whose buffers were reallocated are dropped when next seen).
+int Engine::launch_call_graph(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_on, bool predict, int sblocks,
+                              bool rows) {
+  const GraphKey key{grid_ni, nj, cfg.P, cfg.S, cfg.jchunk, cfg.smajor, cfg.pipe, ngb_on ? 1 : 0,
+                     predict ? 1 : 0, sblocks * 2 + (rows ? 1 : 0), gen_};
+  for (auto& e : graph_cache_) {
+    if (e.key == key) {
+      e.used = ++tick_;
+      HIPCHK(hipGraphLaunch(e.exec, stream_), "graph launch");
+      return 0;
+    }
+  }
+  bool stale = false;
+  for (auto& e : graph_cache_) stale |= e.key.gen != gen_;
+  if (stale || graph_cache_.size() >= MAX_GRAPHS) {
+    HIPCHK(hipStreamSynchronize(stream_), "graph cache sync");
+    for (size_t k = 0; k < graph_cache_.size();) {
+      if (graph_cache_[k].key.gen != gen_) {
+        (void)hipGraphExecDestroy(graph_cache_[k].exec);
+        graph_cache_.erase(graph_cache_.begin() + (long)k);
+      } else {
+        ++k;
+      }
+    }
+    if (graph_cache_.size() >= MAX_GRAPHS) {
+      size_t lru = 0;
+      for (size_t k = 1; k < graph_cache_.size(); ++k)
+        if (graph_cache_[k].used < graph_cache_[lru].used) lru = k;
+      (void)hipGraphExecDestroy(graph_cache_[lru].exec);
+      graph_cache_.erase(graph_cache_.begin() + (long)lru);
+    }
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), "graph capture");
+  capturing_ = true;
+  const int rc = enqueue_call(grid_ni, nj, cfg, ngb_on, predict, sblocks, rows);
+  capturing_ = false;
+  const hipError_t ec = hipStreamEndCapture(stream_, &g);
+  if (rc != 0 || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc != 0 ? rc : fail("graph capture", ec);
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail("graph instantiate", ei);
+  graph_cache_.push_back(GraphEntry{key, x, ++tick_});
+  HIPCHK(hipGraphLaunch(x, stream_), "graph launch");
+  return 0;
+}
+
+void Engine::drop_graphs() {
+  if (graph_cache_.empty()) return;
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& e : graph_cache_) (void)hipGraphExecDestroy(e.exec);
+  graph_cache_.clear();
+}
+
 int Engine::wait_results() {
   if (!inflight_) return 0;
   HIPCHK(hipEventSynchronize(ev_done_), "result wait");
@@ -793,7 +928,7 @@ int Engine::wait_results() {
 }
 
 float Engine::last_force_ms() {
-  if (!timing_ || !have_results_) return 0.f;
+  if (!timing_ || !have_results_ || !timed_force_) return 0.f;
   (void)hipEventSynchronize(ev_f1_);
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, ev_f0_, ev_f1_) != hipSuccess) return 0.f;

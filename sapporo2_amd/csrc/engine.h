@@ -7,7 +7,10 @@
 //     in pinned host memory and sent with one asynchronous copy per flush,
 //     then scattered by a kernel (no per-particle map lookups on the device);
 //   * each force call is scatter -> [predict] -> force -> finalize on one HIP
-//     stream, followed by one asynchronous result copy; lasthalf waits.
+//     stream, followed by one asynchronous result copy; lasthalf waits.  The
+//     kernels take the call's values (block time, softening, counts) from a
+//     small device-side descriptor, so with graphs on the sequence is captured
+//     once per launch shape and replayed (one graph launch per call).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -80,6 +83,10 @@ class Engine {
   int tuning_entries() const { return (int)tune_.size(); }
   const char* tuning_path() const { return tune_path_.c_str(); }
   void set_timing(bool on) { timing_ = on; }
+  // Capture scatter..finalize as a hipGraph per launch shape and replay it.
+  void set_graphs(bool on) { graphs_ = on; }
+  bool graphs() const { return graphs_; }
+  int graph_count() const { return (int)graph_cache_.size(); }
   float last_force_ms();          // force kernel time of the last call (timing on)
   float last_predict_ms();        // predictor time of the last call (timing on, 0 if none)
   float last_device_ms();         // scatter..finalize device time of the last call (timing on)
@@ -98,6 +105,13 @@ class Engine {
   int upload_updates(int* nupd);
   int scatter_updates(int n, int ni_reset);
   int predict_range(int j0, int j1);
+  int post_desc(int ni, int nupd, int ni_reset, double eps2);
+  int scatter_blocks(int nupd, int ni_reset) const;
+  int enqueue_call(int grid_ni, int nj, const sapk::ForceConfig& cfg, bool ngb_on, bool predict,
+                   int sblocks, bool rows);
+  int launch_call_graph(int grid_ni, int nj, const sapk::ForceConfig& cfg, bool ngb_on, bool predict,
+                        int sblocks, bool rows);
+  void drop_graphs();
   int wait_results();
   int fetch_j_host(int nj);
   int grow_staging(int need);
@@ -144,11 +158,32 @@ class Engine {
   int* off_h_ = nullptr; int* packed_h_ = nullptr;
   hipEvent_t ev_done_ = nullptr, ev_d0_ = nullptr, ev_d1_ = nullptr, ev_f0_ = nullptr, ev_f1_ = nullptr, ev_p0_ = nullptr, ev_p1_ = nullptr;
   bool inflight_ = false, have_results_ = false, ngb_fetched_ = false, ngb_on_last_ = false;
-  bool timed_predict_ = false;
+  bool timed_predict_ = false, timed_force_ = false;
+  bool capturing_ = false;
   int ni_last_ = 0, nj_last_ = 0;
   sapk::ForceConfig last_cfg_{64, 1, 1, 0, 0};
   bool timing_ = false;
   int ovr_P_ = 0, ovr_S_ = 0, ovr_smajor_ = 0, ovr_pipe_ = 0;
+
+  // per-call descriptor (pinned host copy + device copy) and graph cache
+  sapk::CallDesc* desc_h_ = nullptr;
+  sapk::CallDesc* desc_d_ = nullptr;
+  hipEvent_t ev_desc_ = nullptr;
+  bool desc_pending_ = false;
+  bool graphs_ = false;
+  uint64_t gen_ = 1;          // bumped whenever a buffer a captured graph uses is reallocated
+  uint64_t tick_ = 0;
+  struct GraphKey {
+    int grid_ni, nj, P, S, jchunk, smajor, pipe, ngb_on, predict, sblocks;
+    uint64_t gen;
+    bool operator==(const GraphKey& o) const {
+      return grid_ni == o.grid_ni && nj == o.nj && P == o.P && S == o.S && jchunk == o.jchunk &&
+             smajor == o.smajor && pipe == o.pipe && ngb_on == o.ngb_on && predict == o.predict &&
+             sblocks == o.sblocks && gen == o.gen;
+    }
+  };
+  struct GraphEntry { GraphKey key; hipGraphExec_t exec; uint64_t used; };
+  std::vector<GraphEntry> graph_cache_;
 
   // ensembles
   int icap_ = 0;

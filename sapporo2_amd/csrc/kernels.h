@@ -73,34 +73,44 @@ struct StoreView {
   int cap;           // multiple of JTILE
 };
 
+// Per-call values the kernels read from device memory, so that a launch
+// sequence does not depend on them and can be captured once and replayed.
+struct CallDesc {
+  double ti;             // block time of the predictor
+  double eps2;           // softening of the force call
+  int ni;                // active i-particles of the call
+  int nupd;              // staged j-updates to scatter
+  int ni_reset;          // neighbour counters to clear before the call
+  int pad;
+};
+
 // Scatter staged updates into the store and reset the per-i neighbour
-// counters of the coming force call (fused: one launch).
+// counters of the coming force call (fused: one launch, grid-stride over
+// cd->nupd and cd->ni_reset).
 struct ScatterArgs {
   StoreView st;
   const double* upd;     // nupd x nf doubles (fields 0..nf-1 of RawField)
   const int* upd_ai;     // nupd x 2 ints: address, id
-  int nupd, nf;
+  int nf;
   int* ngb_cnt;          // zeroed for [0, ni_reset)
-  int ni_reset;
+  const CallDesc* cd;
+  int grid_blocks;       // launch size (host)
 };
 
 struct PredictArgs {
   StoreView st;
   int j0, j1;            // predict addresses [j0, j1)
-  double ti;
+  const CallDesc* cd;    // ti
 };
 
 struct ForceArgs {
   const void* rec;       // predicted records
   const int2* idr;       // tile id ranges (StoreView::idr)
   int nj;
-  const double* ipos;    // ni x 3
-  const double* ivel;    // ni x 3
-  const double* iacc;    // ni x 3 (SIXTH)
-  const double* ih2;     // ni
+  const double* ibuf;    // [xi (ni x 3) | vi (ni x 3) | ai (ni x 3, SIXTH) | h2 (ni)]
   const int* iid;        // ni
-  int ni;
-  double eps2;
+  const CallDesc* cd;    // ni, eps2
+  int grid_ni;           // launch size (host): i-particles the grid covers, >= cd->ni
   double* part;          // part_fields x S x ni FP64 partial sums
   double* part_ds2;      // S x ni  nearest-neighbour ds2 (working precision value)
   int* part_addr;        // S x ni  nearest-neighbour address (INT_MAX: none)
@@ -121,7 +131,10 @@ inline int partial_rows(const ForceConfig& c) { return (c.S % 4 == 0 && !c.smajo
 
 struct FinalizeArgs {
   const double* part; const double* part_ds2; const int* part_addr;
-  int S, ni;
+  int S;
+  const CallDesc* cd;    // ni
+  int grid_ni;           // launch size (host), >= cd->ni
+  int rows;              // shape: 1 thread per i-particle (else a work-group per i-particle)
   const void* rec;       // to look up the neighbour's id
   int* ngb_cnt; int* ngb_list;
   double* out;           // ni x out_fields

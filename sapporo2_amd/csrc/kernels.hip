@@ -69,9 +69,10 @@ template <int ORDER, bool F64>
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
-  for (int k = tid; k < a.ni_reset; k += stride) a.ngb_cnt[k] = 0;
+  const int ni_reset = a.cd->ni_reset, nupd = a.cd->nupd;
+  for (int k = tid; k < ni_reset; k += stride) a.ngb_cnt[k] = 0;
   const size_t cap = (size_t)a.st.cap;
-  for (int k = tid; k < a.nupd; k += stride) {
+  for (int k = tid; k < nupd; k += stride) {
     const double* u = a.upd + (size_t)k * a.nf;
     const int addr = a.upd_ai[2 * k];
     const int id = a.upd_ai[2 * k + 1];
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(256) void k_predict(PredictArgs a) {
   if (j >= a.j1) return;
   const size_t cap = (size_t)a.st.cap;
   double p[3], v[3], ac[3];
-  predict_one<ORDER>(a.st.raw, cap, j, a.ti, p, v, ac);
+  predict_one<ORDER>(a.st.raw, cap, j, a.cd->ti, p, v, ac);
   put_rec<ORDER, F64>(a.st.rec, j, p[0], p[1], p[2], a.st.raw[RF_M * cap + j], v[0], v[1], v[2],
                       ac[0], ac[1], ac[2], a.st.id[j]);
   if constexpr (!F64) {
@@ -455,18 +456,24 @@ __device__ __forceinline__ void wave_force(
 template <int ORDER, bool F64, int P, bool NGB, bool PIPE>
 __global__ __launch_bounds__(256) void k_force(
     const typename RecOf<ORDER, F64>::T* __restrict__ rec, const int2* __restrict__ idr, const int nj,
-    const double* __restrict__ ipos, const double* __restrict__ ivel,
-    const double* __restrict__ iacc, const double* __restrict__ ih2,
-    const int* __restrict__ iid, const int ni, const double eps2d,
+    const double* __restrict__ ibuf, const int* __restrict__ iid, const CallDesc* __restrict__ cd,
     double* __restrict__ part, double* __restrict__ part_ds2, int* __restrict__ part_addr,
     int* __restrict__ ngb_cnt, int* __restrict__ ngb_list, const int S, const int jchunk, const int smajor,
     const int blockred) {
   constexpr int NF = nf<ORDER>();
+  // the call's values come from the descriptor; the grid may cover more
+  // i-groups than there are (a launch shape shared by a range of ni)
+  const int ni = cd->ni;
+  const double eps2d = cd->eps2;
+  const double* ipos = ibuf;
+  const double* ivel = ibuf + 3 * (size_t)ni;
+  const double* iacc = ibuf + 6 * (size_t)ni;
+  const double* ih2 = ibuf + (ORDER >= ORDER_H6 ? 9 : 6) * (size_t)ni;
   // wave-uniform by construction; readfirstlane lets the compiler keep the
   // j-loop bookkeeping and (for P = 64) the j-record loads in SGPRs
   const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int G = (ni + P - 1) / P;
-  if (wave >= G * S) return;  // whole wave
+  if (wave >= G * S) return;  // whole wave (a whole block when S % 4 == 0)
   // smajor: consecutive waves (a block's 4) share one j-range and so its
   // scalar-cache lines; otherwise consecutive waves walk the splits of one group
   const int g = smajor ? wave % G : wave / S;
@@ -578,7 +585,7 @@ template <int ORDER, bool F64>
 __global__ __launch_bounds__(256) void k_finalize_ens(FinalizeArgs f, EnsArgs a) {
   constexpr int NF = nf<ORDER>();
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= f.ni) return;
+  if (i >= f.cd->ni) return;
   const int4 d = a.idesc[i];  // first item, number of splits, i-slot
   double T[NF];
 #pragma unroll
@@ -638,8 +645,9 @@ __device__ __forceinline__ void wave_rank_sort(const int* s, int c, int lane, in
 template <int ORDER, bool F64>
 __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
   constexpr int NF = nf<ORDER>();
+  const int ni = a.cd->ni;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.ni) return;
+  if (i >= ni) return;
   double T[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) T[f] = 0.0;
@@ -650,13 +658,13 @@ __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
   for (; s + 4 <= a.S; s += 4) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      const double* p = a.part + ((size_t)f * a.S + s) * a.ni + i;
-      T[f] += (p[0] + p[a.ni]) + (p[2 * (size_t)a.ni] + p[3 * (size_t)a.ni]);
+      const double* p = a.part + ((size_t)f * a.S + s) * ni + i;
+      T[f] += (p[0] + p[ni]) + (p[2 * (size_t)ni] + p[3 * (size_t)ni]);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double d = a.part_ds2[(size_t)(s + u) * a.ni + i];
-      const int ad2 = a.part_addr[(size_t)(s + u) * a.ni + i];
+      const double d = a.part_ds2[(size_t)(s + u) * ni + i];
+      const int ad2 = a.part_addr[(size_t)(s + u) * ni + i];
       const bool take = (d < ds) || (d == ds && ad2 < ad);
       ds = take ? d : ds;
       ad = take ? ad2 : ad;
@@ -664,9 +672,9 @@ __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
   }
   for (; s < a.S; ++s) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * a.ni + i];
-    const double d = a.part_ds2[(size_t)s * a.ni + i];
-    const int ad2 = a.part_addr[(size_t)s * a.ni + i];
+    for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * ni + i];
+    const double d = a.part_ds2[(size_t)s * ni + i];
+    const int ad2 = a.part_addr[(size_t)s * ni + i];
     const bool take = (d < ds) || (d == ds && ad2 < ad);
     ds = take ? d : ds;
     ad = take ? ad2 : ad;
@@ -680,7 +688,7 @@ __global__ __launch_bounds__(256) void k_sort_lists(FinalizeArgs a) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int i = blockIdx.x * 4 + w;
-  const int c = i < a.ni ? min(a.ngb_cnt[i], NGB_CAP) : 0;
+  const int c = i < a.cd->ni ? min(a.ngb_cnt[i], NGB_CAP) : 0;
   if (c > 1)
     for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
   __syncthreads();
@@ -698,6 +706,8 @@ __global__ __launch_bounds__(256) void k_finalize_block(FinalizeArgs a) {
   __shared__ int lst[NGB_CAP];
   const int t = threadIdx.x;
   const int i = blockIdx.x;
+  const int ni = a.cd->ni;
+  if (i >= ni) return;  // whole block (the grid may cover more i-particles)
   double T[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) T[f] = 0.0;
@@ -705,9 +715,9 @@ __global__ __launch_bounds__(256) void k_finalize_block(FinalizeArgs a) {
   int ad = INT_MAX;
   for (int s = t; s < a.S; s += 256) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * a.ni + i];
-    const double d = a.part_ds2[(size_t)s * a.ni + i];
-    const int ad2 = a.part_addr[(size_t)s * a.ni + i];
+    for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * ni + i];
+    const double d = a.part_ds2[(size_t)s * ni + i];
+    const int ad2 = a.part_addr[(size_t)s * ni + i];
     const bool take = (d < ds) || (d == ds && ad2 < ad);
     ds = take ? d : ds;
     ad = take ? ad2 : ad;
@@ -819,10 +829,8 @@ static void dispatch_op(int order, bool f64, F&& f) {
 }
 
 int launch_scatter(hipStream_t s, int order, bool f64, const ScatterArgs& a) {
-  const long n = a.nupd > a.ni_reset ? a.nupd : a.ni_reset;
-  if (n <= 0) return 0;
-  int nb = blocks_for(n, 256);
-  if (nb > 4096) nb = 4096;
+  if (a.grid_blocks <= 0) return 0;
+  const int nb = a.grid_blocks;  // grid-stride over the descriptor's counts
   dispatch_op(order, f64, [&](auto OR, auto FP) {
     hipLaunchKernelGGL((k_scatter<decltype(OR)::value, decltype(FP)::value>), dim3(nb), dim3(256), 0, s, a);
   });
@@ -845,10 +853,10 @@ bool force_p_supported(int P) {
 template <int OR, bool FP, int P>
 static void launch_force_p(hipStream_t s, const ForceArgs& a, const ForceConfig& c) {
   using Rec = typename RecOf<OR, FP>::T;
-  const long G = (a.ni + P - 1) / P;
+  const long G = (a.grid_ni + P - 1) / P;
   const int nb = blocks_for(G * c.S, 4);
   const Rec* rec = reinterpret_cast<const Rec*>(a.rec);
-#define SAPK_FORCE_ARGS rec, a.idr, a.nj, a.ipos, a.ivel, a.iacc, a.ih2, a.iid, a.ni, a.eps2, a.part, a.part_ds2, \
+#define SAPK_FORCE_ARGS rec, a.idr, a.nj, a.ibuf, a.iid, a.cd, a.part, a.part_ds2, \
                         a.part_addr, a.ngb_cnt, a.ngb_list, c.S, c.jchunk, c.smajor, partial_rows(c) != c.S ? 1 : 0
   if (a.ngb_on && c.pipe)
     hipLaunchKernelGGL((k_force<OR, FP, P, true, true>), dim3(nb), dim3(256), 0, s, SAPK_FORCE_ARGS);
@@ -875,7 +883,7 @@ static int launch_force_t(hipStream_t s, const ForceArgs& a, const ForceConfig& 
 }
 
 int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const ForceConfig& c) {
-  if (a.ni <= 0 || a.nj <= 0) return 0;
+  if (a.grid_ni <= 0 || a.nj <= 0) return 0;
   if (c.S <= 0 || c.jchunk <= 0 || (long)c.S * c.jchunk < a.nj) return -3;
   int rc = 0;
   dispatch_op(order, f64, [&](auto OR, auto FP) { rc = launch_force_t<decltype(OR)::value, decltype(FP)::value>(s, a, c); });
@@ -883,24 +891,23 @@ int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const F
 }
 
 int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a) {
-  if (a.ni <= 0) return 0;
+  if (a.grid_ni <= 0) return 0;
   // rows: coalesced over i, best when many i-particles share few splits
-  const bool rows = a.ni >= 1024;
   dispatch_op(order, f64, [&](auto OR, auto FP) {
     constexpr int O = decltype(OR)::value;
     constexpr bool F = decltype(FP)::value;
-    if (rows) {
-      hipLaunchKernelGGL((k_finalize_rows<O, F>), dim3(blocks_for(a.ni, 256)), dim3(256), 0, s, a);
-      if (a.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(a.ni, 4)), dim3(256), 0, s, a);
+    if (a.rows) {
+      hipLaunchKernelGGL((k_finalize_rows<O, F>), dim3(blocks_for(a.grid_ni, 256)), dim3(256), 0, s, a);
+      if (a.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(a.grid_ni, 4)), dim3(256), 0, s, a);
     } else {
-      hipLaunchKernelGGL((k_finalize_block<O, F>), dim3(a.ni), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_finalize_block<O, F>), dim3(a.grid_ni), dim3(256), 0, s, a);
     }
   });
   return hip_status();
 }
 
 int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int nwaves) {
-  if (a.nitems <= 0 || f.ni <= 0) return 0;
+  if (a.nitems <= 0 || f.grid_ni <= 0) return 0;
   const int nb = blocks_for(nwaves, 4);
   dispatch_op(order, f64, [&](auto OR, auto FP) {
     constexpr int O = decltype(OR)::value;
@@ -909,8 +916,8 @@ int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const
       hipLaunchKernelGGL((k_force_ens<O, F, true>), dim3(nb), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL((k_force_ens<O, F, false>), dim3(nb), dim3(256), 0, s, a);
-    hipLaunchKernelGGL((k_finalize_ens<O, F>), dim3(blocks_for(f.ni, 256)), dim3(256), 0, s, f, a);
-    if (f.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(f.ni, 4)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_finalize_ens<O, F>), dim3(blocks_for(f.grid_ni, 256)), dim3(256), 0, s, f, a);
+    if (f.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(f.grid_ni, 4)), dim3(256), 0, s, f);
   });
   return hip_status();
 }

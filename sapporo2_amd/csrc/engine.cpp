@@ -69,7 +69,7 @@ int Engine::fail(const char* what, hipError_t e) {
 // are compiled in, initial j capacity 16384.
 int Engine::open(int device, int order, int precision) {
   if (open_) close();
-  if (order != ORDER_H4 && order != ORDER_H6) return fail("unsupported integration order");
+  if (order != ORDER_G5 && order != ORDER_H4 && order != ORDER_H6) return fail("unsupported integration order");
   if (precision != PREC_DS && precision != PREC_DOUBLE) return fail("unsupported precision");
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -259,7 +259,7 @@ int Engine::grow_staging(int need) {
 // Reference set_time / set_no_time (sapporohostclass.cpp:405-427).
 int Engine::set_time(double t) {
   t_i_ = t;
-  predict_ = true;
+  predict_ = order_ != ORDER_G5;  // second order has no predictor (kernels.cu:326)
   jhost_valid_ = false;
   return 0;
 }

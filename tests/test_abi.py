@@ -15,6 +15,7 @@ import sapporo2_amd as sa
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include")
 HEADERS = {
+    "g5lib.h": sa.LIB_PATH,
     "g6lib.h": sa.LIB_PATH,
     "sapporo_amd.h": sa.LIB_PATH,
     "sapporo6thlib.h": sa.LIB6_PATH,
@@ -66,7 +67,7 @@ def test_libraries_load_without_gpu():
 
 def test_headers_compile_as_c_and_cxx(tmp_path):
     src_c = tmp_path / "t.c"
-    src_c.write_text('#include "g6lib.h"\n#include "sapporo_amd.h"\n#include "sapporo6thlib.h"\n'
+    src_c.write_text('#include "g5lib.h"\n#include "g6lib.h"\n#include "sapporo_amd.h"\n#include "sapporo6thlib.h"\n'
                      "int main(void){ return 0; }\n")
     subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", INC, str(src_c)])
     src_cc = tmp_path / "t.cpp"

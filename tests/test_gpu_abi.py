@@ -246,6 +246,34 @@ def test_integrator_driver_energy():
     assert finals["double"] < 1e-5 and finals["mixed"] < 1e-5, finals
 
 
+def test_block_timestep_integrator_energy():
+    """The external block-timestep Hermite-4 integrator (individual power-of-two steps,
+    active sets from single particles to all) conserves energy through the g6 ABI."""
+    _say("block-timestep integrator driver")
+    exe = os.path.join(ROOT, "tests", "integrator", "hermite4_block_g6")
+    assert os.path.exists(exe), "build tests/integrator first (__graft_entry__.build)"
+    n = 1024
+    m, x, v = ics.plummer(n, seed=52)
+    path = os.path.join(os.environ.get("TMPDIR", "/tmp"), "h4b_input_%d.bin" % os.getpid())
+    with open(path, "wb") as f:
+        np.array([n], np.int32).tofile(f)
+        np.concatenate([m[:, None], x, v], axis=1).astype(np.float64).tofile(f)
+    finals = {}
+    for prec in ("mixed", "double"):
+        env = dict(os.environ, SAPPORO2_AMD_PRECISION=prec)
+        out = subprocess.run([exe, path, "0.25", "0.02", str(4.0 / n)], capture_output=True, text=True,
+                             timeout=120, env=env)
+        assert out.returncode == 0, out.stderr[-2000:]
+        blk = [l for l in out.stdout.splitlines() if l.startswith("blocks")][-1].split()
+        last = [l for l in out.stdout.splitlines() if l.startswith("final")][-1]
+        finals[prec] = abs(float(last.split()[-1]))
+        _say(f"block integrator {prec}: {' '.join(blk)} {last}")
+        nblocks, mean_active, min_active = int(blk[1]), float(blk[3]), int(blk[5])
+        assert nblocks > 16 and mean_active < n / 2 and min_active < 16, blk   # real block steps
+    os.remove(path)
+    assert finals["double"] < 1e-5 and finals["mixed"] < 1e-5, finals
+
+
 @pytest.mark.parametrize("prec", [sa.DOUBLESINGLE, sa.DOUBLE])
 def test_full_size_properties(prec):
     """N = 131072: bitwise reproducible, Newton's third law, exact NN/lists on a sample."""

@@ -155,6 +155,7 @@ int Engine::close() {
   hfree(off_h_); hfree(packed_h_);
   dfree(gather_d_); gather_cap_ = 0;
   dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
+  dfree(ens_order_d_); dfree(ens_ctr_d_); ens_order_cap_ = 0;
   icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
   for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
     if (*e) (void)hipEventDestroy(*e);
@@ -638,29 +639,46 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   const long target = (long)cus_ * 4 * WAVES_PER_SIMD_TARGET;
   long jch = std::max<long>(MIN_J_PER_SPLIT, (work + target - 1) / target);
   jch = (jch + JTILE * 16 - 1) / (JTILE * 16) * (JTILE * 16);  // whole 64-interaction folds for Q <= 16
+  // items of lane shape P = 64 first, the smaller shapes after them: each part
+  // runs in a kernel of its own, so the P = 64 part keeps its register budget
   ens_items_.clear();
+  std::vector<EnsItem> small;
   ens_idesc_.assign(ni, int4{0, 0, 0, 0});
   for (const Grp& g : grps) {
     const int off = cl_off_[g.c], nj = cl_n_[g.c];
     const int S = std::max(1, (int)((nj + jch - 1) / jch));
+    std::vector<EnsItem>& dst = g.P == 64 ? ens_items_ : small;
     for (int q = 0; q < g.G; ++q) {
-      const int first = (int)ens_items_.size();
+      const int first = (int)dst.size();
       const int i0 = g.ib + q * g.P;
       const int cnt = std::min(g.P, g.n - q * g.P);
       for (int s = 0; s < S; ++s) {
         const int jb = off + (int)std::min<long>((long)s * jch, nj);
         const int je = off + (int)std::min<long>((long)(s + 1) * jch, nj);
-        ens_items_.push_back(EnsItem{i0, cnt, jb, je, g.P, g.c, 0, 0});
+        dst.push_back(EnsItem{i0, cnt, jb, je, g.P, g.c, 0, 0});
       }
-      for (int k = 0; k < cnt; ++k) ens_idesc_[i0 + k] = int4{first, S, k, 0};
+      for (int k = 0; k < cnt; ++k) ens_idesc_[i0 + k] = int4{first, S, k, g.P == 64 ? 0 : 1};
     }
   }
+  const int n64 = (int)ens_items_.size();
+  ens_items_.insert(ens_items_.end(), small.begin(), small.end());
+  for (int i = 0; i < ni; ++i)
+    if (ens_idesc_[i].w) ens_idesc_[i].x += n64;
   const int nitems = (int)ens_items_.size();
+  // queue order: longest j-range first within each part
+  ens_order_.resize(nitems);
+  for (int k = 0; k < nitems; ++k) ens_order_[k] = k;
+  auto longer = [&](int u, int w) { return ens_items_[u].je - ens_items_[u].jb > ens_items_[w].je - ens_items_[w].jb; };
+  std::stable_sort(ens_order_.begin(), ens_order_.begin() + n64, longer);
+  std::stable_sort(ens_order_.begin() + n64, ens_order_.end(), longer);
   if (ens_items_cap_ < nitems) {
     dfree(ens_items_d_);
+    dfree(ens_order_d_);
     ens_items_cap_ = std::max(nitems, 2 * ens_items_cap_);
     HIPCHK(dmalloc(&ens_items_d_, (size_t)ens_items_cap_), "ensemble work table");
+    HIPCHK(dmalloc(&ens_order_d_, (size_t)ens_items_cap_), "ensemble work queue");
   }
+  if (!ens_ctr_d_) HIPCHK(dmalloc(&ens_ctr_d_, (size_t)2), "ensemble queue counters");
   if (ens_eps_cap_ < nc) {
     dfree(ens_eps_d_);
     ens_eps_cap_ = nc;
@@ -668,6 +686,9 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   }
   HIPCHK(hipMemcpyAsync(ens_items_d_, ens_items_.data(), sizeof(EnsItem) * nitems, hipMemcpyHostToDevice, stream_),
          "work table upload");
+  HIPCHK(hipMemcpyAsync(ens_order_d_, ens_order_.data(), sizeof(int) * nitems, hipMemcpyHostToDevice, stream_),
+         "work queue upload");
+  HIPCHK(hipMemsetAsync(ens_ctr_d_, 0, 2 * sizeof(int), stream_), "queue counters");
   HIPCHK(hipMemcpyAsync(idesc_d_, ens_idesc_.data(), sizeof(int4) * ni, hipMemcpyHostToDevice, stream_),
          "descriptor upload");
   HIPCHK(hipMemcpyAsync(ens_eps_d_, eps2c, sizeof(double) * nc, hipMemcpyHostToDevice, stream_), "eps upload");
@@ -695,6 +716,8 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ea.eps2 = ens_eps_d_;
   ea.items = ens_items_d_;
   ea.nitems = nitems;
+  ea.order = ens_order_d_;
+  ea.ctr = ens_ctr_d_;
   ea.idesc = idesc_d_;
   ea.part = part_d_;
   ea.part_ds2 = pds2_d_;
@@ -710,9 +733,8 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   fz.ngb_list = list_d_;
   fz.out = out_d_;
   fz.ngb_on = ea.ngb_on;
-  const int nwaves = std::min<long>(nitems, (long)cus_ * 4 * 8);
   if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
-  if (launch_force_ens(stream_, order_, f64_, ea, fz, nwaves) != 0)
+  if (launch_force_ens(stream_, order_, f64_, ea, fz, n64, (long)cus_ * 4 * 8) != 0)
     return fail("ensemble force launch", hipGetLastError());
   if (timing_) {
     HIPCHK(hipEventRecord(ev_f1_, stream_), "event");

@@ -193,6 +193,9 @@ class Engine {
   std::vector<int4> ens_idesc_;
   sapk::EnsItem* ens_items_d_ = nullptr; int ens_items_cap_ = 0;
   int4* idesc_d_ = nullptr;
+  std::vector<int> ens_order_;
+  int* ens_order_d_ = nullptr; int ens_order_cap_ = 0;
+  int* ens_ctr_d_ = nullptr;
   double* ens_eps_d_ = nullptr; int ens_eps_cap_ = 0;
 
   struct TuneEntry { int order, prec, ngb, li, lj, P, jlog; };

@@ -150,6 +150,8 @@ struct EnsArgs {
   const double* ipos; const double* ivel; const double* iacc; const double* ih2; const int* iid;
   const double* eps2;        // per cluster
   const EnsItem* items; int nitems;
+  const int* order;          // queue: item indices, longest first within each part
+  int* ctr;                  // 2 queue counters (P = 64 part, small-P part), zero at launch
   const int4* idesc;         // per i: first item, number of splits, i-slot
   double* part; double* part_ds2; int* part_addr;   // part_fields x nitems x 64, nitems x 64
   int* ngb_cnt; int* ngb_list; int ngb_on;
@@ -170,8 +172,10 @@ int launch_predict(hipStream_t s, int order, bool f64, const PredictArgs& a);
 int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const ForceConfig& c);
 int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a);
 int launch_pack(hipStream_t s, const PackArgs& a);
-// Ensemble force + finalize (+ list sort) over a work table; nwaves persistent waves.
-int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int nwaves);
+// Ensemble force + finalize (+ list sort) over a work table whose first n64
+// items have lane shape P = 64; at most max_waves persistent waves per part.
+int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64,
+                     long max_waves);
 int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a);
 
 // The P values with compiled force kernels.

@@ -21,6 +21,7 @@
 // Built with -ffp-contract=off: every fused multiply-add below is explicit.
 #include "kernels.h"
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <type_traits>
@@ -306,6 +307,9 @@ __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0,
     nb[u] = pair<ORDER, NGB, CHECK>(r, k0 + u, true, s, eps2, t, dsmin, nnk, ngb_cnt, ngb_list);
   }
   if constexpr (NGB) settle<ORDER>(t, dsmin, nnk);
+  // without lists, keep the scheduler from hoisting the scalar loads of a whole
+  // unrolled sub-block (they exhaust the SGPRs and spill)
+  if constexpr (!NGB) __builtin_amdgcn_sched_barrier(0);
   // the group test is a wave-uniform OR of the four compare masks (a bool OR
   // reduction would be vectorised into per-lane bit arithmetic)
   const bool any = NGB && (__ballot(nb[0]) | __ballot(nb[1]) | __ballot(nb[2]) | __ballot(nb[3])) != 0;
@@ -546,14 +550,14 @@ __device__ __forceinline__ void write_row(const FinalizeArgs& a, int i, const do
 // Ensemble force: a persistent grid of waves walks a work table of items
 // (cluster, i-group, j-split) round-robin; each item carries its own lane shape.
 template <int ORDER, bool F64, int P, bool NGB>
-__device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, int item) {
+__device__ __forceinline__ void ens_item(const EnsArgs& a, const typename RecOf<ORDER, F64>::T* __restrict__ rec,
+                                         const EnsItem& it, int item) {
   constexpr int NF = nf<ORDER>();
-  using Rec = typename RecOf<ORDER, F64>::T;
   double T[NF];
   double dsd;
   int nnaddr, islot;
   bool writer;
-  wave_force<ORDER, F64, P, NGB>(reinterpret_cast<const Rec*>(a.rec), a.idr, a.ipos, a.ivel, a.iacc, a.ih2, a.iid,
+  wave_force<ORDER, F64, P, NGB>(rec, a.idr, a.ipos, a.ivel, a.iacc, a.ih2, a.iid,
                                  it.i0, it.icount, it.jb, it.je, a.eps2[it.cluster], a.ngb_cnt, a.ngb_list, T,
                                  dsd, nnaddr, islot, writer);
   if (writer) {
@@ -565,17 +569,42 @@ __device__ __forceinline__ void ens_item(const EnsArgs& a, const EnsItem& it, in
   }
 }
 
-template <int ORDER, bool F64, bool NGB>
-__global__ __launch_bounds__(256) void k_force_ens(EnsArgs a) {
-  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  const int nwaves = gridDim.x * (blockDim.x >> 6);
-  for (int item = wave; item < a.nitems; item += nwaves) {  // every wave leaves after its last item
-    const EnsItem it = a.items[item];
-    switch (it.P) {
-      case 64: ens_item<ORDER, F64, 64, NGB>(a, it, item); break;
-      case 16: ens_item<ORDER, F64, 16, NGB>(a, it, item); break;
-      case 4: ens_item<ORDER, F64, 4, NGB>(a, it, item); break;
-      default: ens_item<ORDER, F64, 1, NGB>(a, it, item); break;
+// P64: items [ib, ie) all have lane shape 64; otherwise 16, 4 or 1 (a switch
+// whose larger register needs stay out of the P = 64 kernel).
+// Waves take queue positions [ib, ie) from a counter (one vector atomic per
+// item, issued by lane 0); the queue lists the longest items first, so the
+// short ones fill the end of the launch.  Every wave leaves once the counter
+// has passed ie.
+template <int ORDER, bool F64, bool NGB, bool P64>
+__global__ __launch_bounds__(256) void k_force_ens(EnsArgs a, const typename RecOf<ORDER, F64>::T* __restrict__ rec,
+                                                   int ib, int ie, int part) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int k = 0;
+    if (lane == 0) k = atomicAdd(a.ctr + part, 1);
+    const int q = ib + __builtin_amdgcn_readfirstlane(k);
+    if (q >= ie) break;
+    const int item = __builtin_amdgcn_readfirstlane(a.order[q]);
+    // the item is wave-uniform; readfirstlane keeps its fields (and so the
+    // j-record addresses) in SGPRs, which a load the compiler cannot prove
+    // invariant would otherwise place in VGPRs
+    const EnsItem ld = a.items[item];
+    EnsItem it;
+    it.i0 = __builtin_amdgcn_readfirstlane(ld.i0);
+    it.icount = __builtin_amdgcn_readfirstlane(ld.icount);
+    it.jb = __builtin_amdgcn_readfirstlane(ld.jb);
+    it.je = __builtin_amdgcn_readfirstlane(ld.je);
+    it.P = __builtin_amdgcn_readfirstlane(ld.P);
+    it.cluster = __builtin_amdgcn_readfirstlane(ld.cluster);
+    it.pad0 = it.pad1 = 0;
+    if constexpr (P64) {
+      ens_item<ORDER, F64, 64, NGB>(a, rec, it, item);
+    } else {
+      switch (it.P) {
+        case 16: ens_item<ORDER, F64, 16, NGB>(a, rec, it, item); break;
+        case 4: ens_item<ORDER, F64, 4, NGB>(a, rec, it, item); break;
+        default: ens_item<ORDER, F64, 1, NGB>(a, rec, it, item); break;
+      }
     }
   }
 }
@@ -906,16 +935,28 @@ int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a) {
   return hip_status();
 }
 
-int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int nwaves) {
+int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64,
+                     long max_waves) {
   if (a.nitems <= 0 || f.grid_ni <= 0) return 0;
-  const int nb = blocks_for(nwaves, 4);
+  const int nsmall = a.nitems - n64;
+  const int nb64 = blocks_for(std::min<long>(n64, max_waves), 4);
+  const int nbs = blocks_for(std::min<long>(nsmall, max_waves), 4);
   dispatch_op(order, f64, [&](auto OR, auto FP) {
     constexpr int O = decltype(OR)::value;
     constexpr bool F = decltype(FP)::value;
-    if (a.ngb_on)
-      hipLaunchKernelGGL((k_force_ens<O, F, true>), dim3(nb), dim3(256), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_force_ens<O, F, false>), dim3(nb), dim3(256), 0, s, a);
+    const auto* rec = reinterpret_cast<const typename RecOf<O, F>::T*>(a.rec);
+    if (n64 > 0) {
+      if (a.ngb_on)
+        hipLaunchKernelGGL((k_force_ens<O, F, true, true>), dim3(nb64), dim3(256), 0, s, a, rec, 0, n64, 0);
+      else
+        hipLaunchKernelGGL((k_force_ens<O, F, false, true>), dim3(nb64), dim3(256), 0, s, a, rec, 0, n64, 0);
+    }
+    if (nsmall > 0) {
+      if (a.ngb_on)
+        hipLaunchKernelGGL((k_force_ens<O, F, true, false>), dim3(nbs), dim3(256), 0, s, a, rec, n64, a.nitems, 1);
+      else
+        hipLaunchKernelGGL((k_force_ens<O, F, false, false>), dim3(nbs), dim3(256), 0, s, a, rec, n64, a.nitems, 1);
+    }
     hipLaunchKernelGGL((k_finalize_ens<O, F>), dim3(blocks_for(f.grid_ni, 256)), dim3(256), 0, s, f, a);
     if (f.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(f.grid_ni, 4)), dim3(256), 0, s, f);
   });

@@ -671,35 +671,22 @@ __device__ __forceinline__ void wave_rank_sort(const int* s, int c, int lane, in
   }
 }
 
+// Four consecutive lanes per i-particle: lane u sums rows u, u + 4, ... in
+// order, then the four sums combine as (S0 + S1) + (S2 + S3) (IEEE addition
+// commutes exactly, so every lane of the four holds the same value).
 template <int ORDER, bool F64>
 __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
   constexpr int NF = nf<ORDER>();
   const int ni = a.cd->ni;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= ni) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int i = t >> 2, u = t & 3;
+  if (i >= ni) return;  // the four lanes of an i-particle leave together
   double T[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) T[f] = 0.0;
   double ds = INFINITY;
   int ad = INT_MAX;
-  // groups of 4 splits summed as ((p0 + p1) + (p2 + p3)): fixed order, 4x the loads in flight
-  int s = 0;
-  for (; s + 4 <= a.S; s += 4) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      const double* p = a.part + ((size_t)f * a.S + s) * ni + i;
-      T[f] += (p[0] + p[ni]) + (p[2 * (size_t)ni] + p[3 * (size_t)ni]);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double d = a.part_ds2[(size_t)(s + u) * ni + i];
-      const int ad2 = a.part_addr[(size_t)(s + u) * ni + i];
-      const bool take = (d < ds) || (d == ds && ad2 < ad);
-      ds = take ? d : ds;
-      ad = take ? ad2 : ad;
-    }
-  }
-  for (; s < a.S; ++s) {
+  for (int s = u; s < a.S; s += 4) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) T[f] += a.part[((size_t)f * a.S + s) * ni + i];
     const double d = a.part_ds2[(size_t)s * ni + i];
@@ -708,7 +695,17 @@ __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
     ds = take ? d : ds;
     ad = take ? ad2 : ad;
   }
-  write_row<ORDER, F64>(a, i, T, ds, ad, a.ngb_on ? a.ngb_cnt[i] : 0);
+#pragma unroll
+  for (int m = 1; m < 4; m <<= 1) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) T[f] += __shfl_xor(T[f], m);
+    const double od = __shfl_xor(ds, m);
+    const int oa = __shfl_xor(ad, m);
+    const bool take = (od < ds) || (od == ds && oa < ad);
+    ds = take ? od : ds;
+    ad = take ? oa : ad;
+  }
+  if (u == 0) write_row<ORDER, F64>(a, i, T, ds, ad, a.ngb_on ? a.ngb_cnt[i] : 0);
 }
 
 // One wave per i-particle: ascending sort of its (at most NGB_CAP) list.
@@ -926,7 +923,7 @@ int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a) {
     constexpr int O = decltype(OR)::value;
     constexpr bool F = decltype(FP)::value;
     if (a.rows) {
-      hipLaunchKernelGGL((k_finalize_rows<O, F>), dim3(blocks_for(a.grid_ni, 256)), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_finalize_rows<O, F>), dim3(blocks_for(4L * a.grid_ni, 256)), dim3(256), 0, s, a);
       if (a.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(a.grid_ni, 4)), dim3(256), 0, s, a);
     } else {
       hipLaunchKernelGGL((k_finalize_block<O, F>), dim3(a.grid_ni), dim3(256), 0, s, a);

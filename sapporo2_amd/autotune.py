@@ -27,7 +27,11 @@ import sapporo2_amd as sa
 from sapporo2_amd import ics
 
 
-def candidates(ni, nj):
+def candidates(ni, nj, min_waves=128):
+    """Lane shapes P (the active count's power of two and two narrower ones) and
+    power-of-two j-splits S with at least max(64, 16 Q) j per split and between
+    min_waves and 262144 waves; the wave floor is dropped when no shape meets it
+    (few i-particles against few j)."""
     pnat = 1
     while pnat < min(ni, 64):
         pnat <<= 1
@@ -41,9 +45,11 @@ def candidates(ni, nj):
             if jchunk < max(64, 16 * Q):
                 break
             waves = G * S
-            if waves < 128 or waves > 262144:
+            if waves < min_waves or waves > 262144:
                 continue
             out.append((P, S))
+    if not out and min_waves > 1:
+        return candidates(ni, nj, 1)
     return out
 
 

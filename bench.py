@@ -166,7 +166,7 @@ def summarize(recs, w, prec, steps):
 
 def cpu_baseline_and_accuracy(w, kept_mix, kept_d, nthreads):
     """FP64 oracle on the kept sample (timed on the host cores) and the accuracy of
-    both GPU kernels against it (the oracle is the checker here)."""
+    both GPU kernels against it (the oracle's results are the reference)."""
     import oracle
     smp = kept_mix["sample"]
     sel = smp["sel"]

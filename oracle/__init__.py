@@ -1,6 +1,6 @@
 """CPU oracle for the direct-summation hot path.
 
-TEST INFRASTRUCTURE ONLY: the checker used by ``tests/``, ``__graft_entry__.smoke()``
+TEST INFRASTRUCTURE ONLY: the reference results for ``tests/``, ``__graft_entry__.smoke()``
 and the ``cpu_baseline`` leg of ``bench.py``.  The product library never imports it.
 
 Thin ctypes bindings over ``oracle/liboracle.so`` (built from ``oracle.c`` by the

@@ -1,8 +1,8 @@
 /*
  * oracle.c -- CPU restatement of the reference's direct-summation hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
- * smoke() entry and bench.py's cpu_baseline leg may load it.  The product
+ * TEST INFRASTRUCTURE ONLY.  The results of the GPU path are compared with this
+ * file's: only tests/, the smoke() entry and bench.py's cpu_baseline leg may load it.  The product
  * library (sapporo2_amd/) never links or calls it.
  *
  * Contents (each restates the reference file:line named beside it):

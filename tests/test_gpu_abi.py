@@ -1,5 +1,5 @@
 """GPU tests through the drop-in C-ABIs themselves (GRAPE-6 g6_* in libsapporo_amd.so,
-6th-order in libsapporo6th_amd.so), their error paths, the external integrator driver,
+6th-order in libsapporo6th_amd.so), their error paths, the external integrator programs,
 and size-independent properties at the full N = 131072."""
 from __future__ import annotations
 
@@ -224,7 +224,7 @@ def test_sixth_order_abi():
 
 def test_integrator_driver_energy():
     """The external GRAPE-6 Hermite-4 integrator, linked unchanged, conserves energy."""
-    _say("integrator driver")
+    _say("shared-step integrator")
     exe = os.path.join(ROOT, "tests", "integrator", "hermite4_g6")
     assert os.path.exists(exe), "build tests/integrator first (__graft_entry__.build)"
     n = 1024
@@ -249,7 +249,7 @@ def test_integrator_driver_energy():
 def test_block_timestep_integrator_energy():
     """The external block-timestep Hermite-4 integrator (individual power-of-two steps,
     active sets from single particles to all) conserves energy through the g6 ABI."""
-    _say("block-timestep integrator driver")
+    _say("block-timestep integrator")
     exe = os.path.join(ROOT, "tests", "integrator", "hermite4_block_g6")
     assert os.path.exists(exe), "build tests/integrator first (__graft_entry__.build)"
     n = 1024

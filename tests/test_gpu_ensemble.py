@@ -83,3 +83,57 @@ def test_ensemble_matches_clusters_alone(prec, local_ids):
             bad[[0, -1]] = bad[[-1, 0]]
             with pytest.raises(RuntimeError):
                 eng.start_grav_calc_clusters(bad, I, X, V, eps2, H)
+
+
+@pytest.mark.parametrize("prec", [sa.DOUBLESINGLE, sa.DOUBLE])
+def test_ensemble_h6_predicted_no_lists(prec):
+    """Hermite-6 ensemble on predicted j-particles without neighbour lists: snap and the
+    nearest neighbour per cluster against the oracle on the predicted state."""
+    _say(f"ensemble h6 prec={prec}")
+    rng = np.random.Generator(np.random.PCG64(75))
+    sizes = ics.ensemble_sizes(12, 256, 2048, seed=76)
+    eps2 = rng.choice([1e-4, 1e-3], len(sizes))
+    ti = 1.0 / 1024
+    cl = []
+    for c, n in enumerate(sizes):
+        m, x, v = ics.plummer(int(n), seed=200 + c)
+        ids = np.arange(n, dtype=np.int32)
+        acc, jrk, snp, crk = (rng.normal(0, 0.3, (n, 3)) for _ in range(4))
+        tj = -rng.uniform(0, 1.0 / 512, n)
+        cl.append((m, x, v, ids, acc, jrk, snp, crk, tj))
+    with sa.Engine(sa.SIXTH, prec) as eng:
+        offs = eng.set_clusters(sizes)
+        for c, (m, x, v, ids, acc, jrk, snp, crk, tj) in enumerate(cl):
+            eng.set_j_particles(offs[c] + np.arange(len(m)), ids, m, x, v, acc, jrk, snp, crk, tj=tj)
+        eng.set_time(ti)
+        pick, pred = [], []
+        for c, (m, x, v, ids, acc, jrk, snp, crk, tj) in enumerate(cl):
+            n = len(m)
+            k = [1, 5, n // 2, n][c % 4]
+            pick.append(np.sort(rng.choice(n, k, replace=False)))
+            pred.append(oracle.predict(2, ti, tj, x, v, acc, jrk, snp, crk))
+        cidx = np.concatenate([np.full(len(p), c, np.int32) for c, p in enumerate(pick)])
+        X = np.concatenate([pred[c][0][p] for c, p in enumerate(pick)])
+        V = np.concatenate([pred[c][1][p] for c, p in enumerate(pick)])
+        A = np.concatenate([pred[c][2][p] for c, p in enumerate(pick)])
+        I = np.concatenate([cl[c][3][p] for c, p in enumerate(pick)])
+        eng.start_grav_calc_clusters(cidx, I, X, V, eps2, None, ai=A)
+        r = eng.get_grav_results(1, len(I))
+        o = 0
+        tol = 1e-6 if prec == sa.DOUBLESINGLE else 1e-12
+        for c, p in enumerate(pick):
+            k = len(p)
+            m, ids = cl[c][0], cl[c][3]
+            px, pv, pa = pred[c]
+            sl = slice(o, o + k)
+            ref = oracle.force(2, px[p], pv[p], ids[p], px, pv, m, ids, eps2[c], ai=pa[p], aj=pa)
+            sub = {kk: r[kk][sl] for kk in ("acc", "jrk", "snp", "pot")}
+            errs = check_forces(sub, ref, 2, tol * (10 if k < 5 else 1), label=f"h6 cluster {c}")
+            if prec == sa.DOUBLE:
+                assert errs["snp"][0] < 1e-10, (c, errs["snp"])
+            nn = ref if prec == sa.DOUBLE else oracle.neighbours_mixed(px[p], ids[p], px, ids, np.zeros(k))
+            nnb, dsmin, found = oracle.api_nn(nn["nn_addr"], nn["nn_ds2"], ids)
+            assert np.array_equal(r["nnb"][sl], nnb), f"h6 cluster {c} nnb"
+            assert np.array_equal(r["dsmin"][sl][found], dsmin[found]), f"h6 cluster {c} dsmin"
+            o += k
+        assert o == len(I)

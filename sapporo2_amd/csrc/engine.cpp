@@ -586,6 +586,31 @@ int Engine::cluster_offset(int c) const {
 // Forces for i-particles of many clusters in one launch.  The i-particles
 // are grouped by cluster (non-decreasing cluster index); each interacts with
 // its own cluster's j-particles only, with that cluster's eps2.
+// Packs the i-particles into the pinned upload buffer: [xi | vi | (ai) | h2]
+// doubles and ids (reference sapporohostclass.cpp:613-649); returns whether
+// any h2 > 0 (neighbour search on).
+bool Engine::pack_i(int ni, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
+                    const double* h2, double* bx, double* bv, double* ba, double* bh) {
+  const size_t n3 = 3 * (size_t)ni;
+  memcpy(bx, xi, sizeof(double) * n3);
+  if (vi) memcpy(bv, vi, sizeof(double) * n3);
+  else memset(bv, 0, sizeof(double) * n3);
+  if (order_ >= ORDER_H6) {
+    if (ai) memcpy(ba, ai, sizeof(double) * n3);
+    else memset(ba, 0, sizeof(double) * n3);
+  }
+  bool ngb_on = false;
+  if (h2) {
+    memcpy(bh, h2, sizeof(double) * (size_t)ni);
+    for (int i = 0; i < ni; ++i) ngb_on |= h2[i] > 0.0;
+  } else {
+    memset(bh, 0, sizeof(double) * (size_t)ni);
+  }
+  if (id) memcpy(iid_h_, id, sizeof(int) * (size_t)ni);
+  else std::fill(iid_h_, iid_h_ + ni, -1);
+  return ngb_on;
+}
+
 int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, const double (*xi)[3],
                                      const double (*vi)[3], const double (*ai)[3], const double* eps2c,
                                      const double* h2) {
@@ -606,18 +631,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   double* bv = bx + 3 * (size_t)ni;
   double* ba = bv + 3 * (size_t)ni;
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
-  bool ngb_on = false;
-  for (int i = 0; i < ni; ++i) {
-    for (int d = 0; d < 3; ++d) {
-      bx[3 * i + d] = xi[i][d];
-      bv[3 * i + d] = vi ? vi[i][d] : 0.0;
-      if (order_ >= ORDER_H6) ba[3 * i + d] = ai ? ai[i][d] : 0.0;
-    }
-    const double hh = h2 ? h2[i] : 0.0;
-    bh[i] = hh;
-    ngb_on |= hh > 0.0;
-    iid_h_[i] = id ? id[i] : -1;
-  }
+  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh);
   const size_t nd = (size_t)(bh + ni - bx);
   HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
   HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
@@ -772,18 +786,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   double* bv = bx + 3 * (size_t)ni;
   double* ba = bv + 3 * (size_t)ni;
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
-  bool ngb_on = false;
-  for (int i = 0; i < ni; ++i) {
-    for (int d = 0; d < 3; ++d) {
-      bx[3 * i + d] = xi[i][d];
-      bv[3 * i + d] = vi ? vi[i][d] : 0.0;
-      if (order_ >= ORDER_H6) ba[3 * i + d] = ai ? ai[i][d] : 0.0;
-    }
-    const double hh = h2 ? h2[i] : 0.0;
-    bh[i] = hh;
-    ngb_on |= hh > 0.0;
-    iid_h_[i] = id ? id[i] : -1;
-  }
+  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh);
   const size_t nd = (size_t)(bh + ni - bx);
   HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
   HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");

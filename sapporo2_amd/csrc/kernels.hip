@@ -675,10 +675,10 @@ __device__ __forceinline__ void wave_rank_sort(const int* s, int c, int lane, in
 // order, then the four sums combine as (S0 + S1) + (S2 + S3) (IEEE addition
 // commutes exactly, so every lane of the four holds the same value).
 template <int ORDER, bool F64>
-__global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
+__device__ __forceinline__ void finalize_rows_block(const FinalizeArgs& a, int b) {
   constexpr int NF = nf<ORDER>();
   const int ni = a.cd->ni;
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = b * 256 + threadIdx.x;
   const int i = t >> 2, u = t & 3;
   if (i >= ni) return;  // the four lanes of an i-particle leave together
   double T[NF];
@@ -708,17 +708,33 @@ __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
   if (u == 0) write_row<ORDER, F64>(a, i, T, ds, ad, a.ngb_on ? a.ngb_cnt[i] : 0);
 }
 
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
+  finalize_rows_block<ORDER, F64>(a, blockIdx.x);
+}
+
 // One wave per i-particle: ascending sort of its (at most NGB_CAP) list.
-__global__ __launch_bounds__(256) void k_sort_lists(FinalizeArgs a) {
+__device__ __forceinline__ void sort_lists_block(const FinalizeArgs& a, int b) {
   __shared__ int lst[4][NGB_CAP];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int i = blockIdx.x * 4 + w;
+  const int i = b * 4 + w;
   const int c = i < a.cd->ni ? min(a.ngb_cnt[i], NGB_CAP) : 0;
   if (c > 1)
     for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
   __syncthreads();
   if (c > 1) wave_rank_sort(lst[w], c, lane, a.ngb_list + (size_t)i * NGB_CAP);
+}
+
+__global__ __launch_bounds__(256) void k_sort_lists(FinalizeArgs a) { sort_lists_block(a, blockIdx.x); }
+
+// Finalize and list sort in one launch (independent work: the sort only
+// reorders the lists, the finalize reads counts): the first nfin work-groups
+// finalize, the others sort; both roles are uniform per work-group.
+template <int ORDER, bool F64>
+__global__ __launch_bounds__(256) void k_finalize_rows_sort(FinalizeArgs a, int nfin) {
+  if ((int)blockIdx.x < nfin) finalize_rows_block<ORDER, F64>(a, blockIdx.x);
+  else sort_lists_block(a, blockIdx.x - nfin);
 }
 
 // One 256-thread block per i-particle: thread t sums the splits s = t, t+256, ...
@@ -923,8 +939,12 @@ int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a) {
     constexpr int O = decltype(OR)::value;
     constexpr bool F = decltype(FP)::value;
     if (a.rows) {
-      hipLaunchKernelGGL((k_finalize_rows<O, F>), dim3(blocks_for(4L * a.grid_ni, 256)), dim3(256), 0, s, a);
-      if (a.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(a.grid_ni, 4)), dim3(256), 0, s, a);
+      const int nfin = blocks_for(4L * a.grid_ni, 256);
+      if (a.ngb_on)
+        hipLaunchKernelGGL((k_finalize_rows_sort<O, F>), dim3(nfin + blocks_for(a.grid_ni, 4)), dim3(256), 0, s, a,
+                           nfin);
+      else
+        hipLaunchKernelGGL((k_finalize_rows<O, F>), dim3(nfin), dim3(256), 0, s, a);
     } else {
       hipLaunchKernelGGL((k_finalize_block<O, F>), dim3(a.grid_ni), dim3(256), 0, s, a);
     }

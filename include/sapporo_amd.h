@@ -104,6 +104,13 @@ int sapamd_set_timing(sapamd_engine *e, int on);
  * No reference counterpart (the reference launches each kernel, sapporohostclass.cpp:1442-1602).
  * Default: off, or $SAPPORO2_AMD_GRAPHS. */
 int sapamd_set_graphs(sapamd_engine *e, int on);
+/* Array form of sapamd_retrieve_predicted_j_particle for n addresses: mass,
+ * predicted position, velocity (and acceleration, SIXTH; zero otherwise) of
+ * each, gathered on the device so that only those rows cross PCIe.  No
+ * counterpart in the reference, which copies every j back
+ * (sapporohostclass.cpp:980-995).  Any output may be NULL. */
+int sapamd_retrieve_predicted_j_particles(sapamd_engine *e, int n, const int *addr, double *mass,
+                                          double (*pos)[3], double (*vel)[3], double (*acc)[3]);
 int sapamd_graph_count(sapamd_engine *e);
 float sapamd_last_force_ms(sapamd_engine *e);
 float sapamd_last_predict_ms(sapamd_engine *e);

@@ -51,7 +51,7 @@ ENGINE_SYMBOLS = [
     "sapamd_set_time", "sapamd_set_no_time", "sapamd_set_j_particle", "sapamd_start_grav_calc",
     "sapamd_get_grav_results", "sapamd_set_j_particles", "sapamd_last_device_ms", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
-    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing", "sapamd_set_graphs", "sapamd_graph_count",
+    "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing", "sapamd_set_graphs", "sapamd_graph_count", "sapamd_retrieve_predicted_j_particles",
     "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
     "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
     "sapamd_start_grav_calc_clusters",
@@ -96,6 +96,7 @@ def lib():
         L.sapamd_set_force_override.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.sapamd_set_timing.argtypes = [_vp, ctypes.c_int]
         L.sapamd_set_graphs.argtypes = [_vp, ctypes.c_int]
+        L.sapamd_retrieve_predicted_j_particles.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
         L.sapamd_graph_count.argtypes = [_vp]
         L.sapamd_set_split_major.argtypes = [_vp, ctypes.c_int]
         L.sapamd_set_ngb_pipeline.argtypes = [_vp, ctypes.c_int]
@@ -283,6 +284,15 @@ class Engine:
             self._h, int(addr), ctypes.byref(m), ctypes.byref(idd), ctypes.byref(e2), _pd(pos),
             _pd(vel), _pd(acc)), "retrieve_predicted_j_particle")
         return dict(mass=m.value, id=int(idd.value), eps2=e2.value, pos=pos, vel=vel, acc=acc)
+
+    def retrieve_predicted_j_particles(self, addresses):
+        """Array form: mass, predicted pos, vel, acc of the listed addresses (device gather)."""
+        a = np.ascontiguousarray(addresses, np.int32)
+        n = len(a)
+        m = np.zeros(n); pos = np.zeros((n, 3)); vel = np.zeros((n, 3)); acc = np.zeros((n, 3))
+        self._check(self._L.sapamd_retrieve_predicted_j_particles(self._h, n, _pi(a), _pd(m), _pd(pos), _pd(vel),
+                                                                  _pd(acc)), "retrieve_predicted_j_particles")
+        return dict(mass=m, pos=pos, vel=vel, acc=acc)
 
     def retrieve_j_particle_state(self, addr):
         m = ctypes.c_double(); idd = ctypes.c_double(); e2 = ctypes.c_double()

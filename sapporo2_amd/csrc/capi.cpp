@@ -101,6 +101,12 @@ int sapamd_set_timing(sapamd_engine* e, int on) {
   return 0;
 }
 
+int sapamd_retrieve_predicted_j_particles(sapamd_engine* e, int n, const int* addr, double* mass,
+                                          double (*pos)[3], double (*vel)[3], double (*acc)[3]) {
+  if (!e) return -1;
+  return e->eng.retrieve_predicted_j_particles(n, addr, mass, pos, vel, acc);
+}
+
 int sapamd_set_graphs(sapamd_engine* e, int on) {
   if (!e) return -1;
   e->eng.set_graphs(on != 0);

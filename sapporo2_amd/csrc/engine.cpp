@@ -154,6 +154,7 @@ int Engine::close() {
   dfree(cnt_d_); dfree(list_d_); dfree(off_d_); dfree(packed_d_);
   hfree(off_h_); hfree(packed_h_);
   dfree(gather_d_); gather_cap_ = 0;
+  dfree(gaddr_d_); gaddr_cap_ = 0;
   dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
   dfree(ens_order_d_); dfree(ens_ctr_d_); ens_order_cap_ = 0;
   icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
@@ -1098,6 +1099,7 @@ int Engine::fetch_j_host(int need) {
   g.st = st_;
   g.n = n;
   g.out = gather_d_;
+  g.addr = nullptr;
   if (launch_gather_pred(stream_, order_, f64_, g) != 0) return fail("gather launch", hipGetLastError());
   const int nfr = raw_fields(order_);
   jpred_h_.resize((size_t)n * 10);
@@ -1117,6 +1119,47 @@ int Engine::fetch_j_host(int need) {
 }
 
 // Reference retrieve_predicted_j_particle (sapporohostclass.cpp:969-1041).
+int Engine::retrieve_predicted_j_particles(int n, const int* addr, double* mass, double (*pos)[3],
+                                           double (*vel)[3], double (*acc)[3]) {
+  if (!open_) return fail("retrieve_predicted_j_particles: library not open");
+  if (n < 0 || (n > 0 && !addr)) return fail("retrieve_predicted_j_particles: bad arguments");
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (inflight_ && wait_results() != 0) return -1;
+  if (flush_updates(0) != 0) return -1;
+  for (int k = 0; k < n; ++k)
+    if (addr[k] < 0 || addr[k] > max_addr_) return fail("retrieve_predicted_j_particles: address never set");
+  if (gather_cap_ < (size_t)n) {
+    dfree(gather_d_);
+    HIPCHK(dmalloc(&gather_d_, (size_t)n * 10), "gather buffer");
+    gather_cap_ = n;
+  }
+  if (gaddr_cap_ < (size_t)n) {
+    dfree(gaddr_d_);
+    HIPCHK(dmalloc(&gaddr_d_, (size_t)n), "gather addresses");
+    gaddr_cap_ = n;
+  }
+  HIPCHK(hipMemcpyAsync(gaddr_d_, addr, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, stream_), "address upload");
+  GatherArgs g{};
+  g.st = st_;
+  g.n = n;
+  g.out = gather_d_;
+  g.addr = gaddr_d_;
+  if (launch_gather_pred(stream_, order_, f64_, g) != 0) return fail("gather launch", hipGetLastError());
+  gather_h_.resize((size_t)n * 10);
+  HIPCHK(hipMemcpyAsync(gather_h_.data(), gather_d_, sizeof(double) * (size_t)n * 10, hipMemcpyDeviceToHost,
+                        stream_), "gather download");
+  HIPCHK(hipStreamSynchronize(stream_), "read-back sync");
+  for (int k = 0; k < n; ++k) {
+    const double* p = gather_h_.data() + (size_t)k * 10;
+    if (pos) { pos[k][0] = p[0]; pos[k][1] = p[1]; pos[k][2] = p[2]; }
+    if (mass) mass[k] = p[3];
+    if (vel) { vel[k][0] = p[4]; vel[k][1] = p[5]; vel[k][2] = p[6]; }
+    if (acc) { acc[k][0] = p[7]; acc[k][1] = p[8]; acc[k][2] = p[9]; }
+  }
+  return 0;
+}
+
 int Engine::retrieve_predicted_j_particle(int addr, double* mass, double* id, double* eps2, double pos[3],
                                           double vel[3], double acc[3]) {
   if (!open_) return fail("retrieve_predicted_j_particle: library not open");

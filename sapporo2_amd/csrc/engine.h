@@ -68,6 +68,11 @@ class Engine {
   int force_prediction(int nj);
   int retrieve_predicted_j_particle(int addr, double* mass, double* id, double* eps2, double pos[3],
                                     double vel[3], double acc[3]);
+  // Array form of retrieve_predicted_j_particle for n addresses: the rows are
+  // gathered on the device and only they are copied back (no counterpart in the
+  // reference, whose read-back copies every j, sapporohostclass.cpp:980-995).
+  int retrieve_predicted_j_particles(int n, const int* addr, double* mass, double (*pos)[3], double (*vel)[3],
+                                     double (*acc)[3]);
   int retrieve_j_particle_state(int addr, double* mass, double* id, double* eps2, double pos[3],
                                 double vel[3], double acc[3], double jrk[3], double ppos[3],
                                 double pvel[3], double pacc[3]);
@@ -210,6 +215,8 @@ class Engine {
   std::vector<double> jpred_h_, jraw_h_;
   std::vector<int> jid_h_;
   double* gather_d_ = nullptr; size_t gather_cap_ = 0;
+  int* gaddr_d_ = nullptr; size_t gaddr_cap_ = 0;
+  std::vector<double> gather_h_;
 };
 
 }  // namespace sap

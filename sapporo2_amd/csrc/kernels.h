@@ -163,9 +163,10 @@ struct PackArgs {
   int* packed;           // sum(min(cnt, NGB_CAP))
 };
 
-// Gathers the FP64 predicted state (pos, vel, [acc]) of all j in [0, n) into
-// a dense AoS buffer for read-back: 3 + 3 (+3) doubles plus mass per j.
-struct GatherArgs { StoreView st; int n; double* out; };
+// Gathers the FP64 predicted state (pos, mass, vel, [acc]) into a dense AoS
+// buffer for read-back, 10 doubles per row: of all j in [0, n) (addr null) or
+// of the n listed addresses.
+struct GatherArgs { StoreView st; int n; double* out; const int* addr; };
 
 int launch_scatter(hipStream_t s, int order, bool f64, const ScatterArgs& a);
 int launch_predict(hipStream_t s, int order, bool f64, const PredictArgs& a);

@@ -834,11 +834,12 @@ __global__ __launch_bounds__(256) void k_pack_copy(PackArgs a) {
 template <int ORDER, bool F64>
 __global__ __launch_bounds__(256) void k_gather_pred(GatherArgs a) {
   using Rec = typename RecOf<ORDER, F64>::T;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.n) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const int j = a.addr ? a.addr[k] : k;
   const Rec r = reinterpret_cast<const Rec*>(a.st.rec)[j];
   const size_t cap = (size_t)a.st.cap;
-  double* o = a.out + (size_t)j * 10;
+  double* o = a.out + (size_t)k * 10;
   o[3] = a.st.raw[RF_M * cap + j];
   if constexpr (F64) {
     o[0] = r.x; o[1] = r.y; o[2] = r.z;

@@ -104,6 +104,17 @@ int sapamd_set_timing(sapamd_engine *e, int on);
  * No reference counterpart (the reference launches each kernel, sapporohostclass.cpp:1442-1602).
  * Default: off, or $SAPPORO2_AMD_GRAPHS. */
 int sapamd_set_graphs(sapamd_engine *e, int on);
+/* Checkpoint of the j-store, no counterpart in the reference (whose state
+ * lives only in the process): sapamd_state_bytes flushes staged updates and
+ * returns the size of a dump (-1 on error); sapamd_save_state writes it into a
+ * caller-owned host buffer of at least that size; sapamd_load_state replaces
+ * the whole store of a handle opened with the same order and precision (stored
+ * fields, predicted state, block time, pending prediction, cluster layout), so
+ * that the next calls give bit-identical results.  Results and neighbour lists
+ * of earlier calls are not part of the dump. */
+long long sapamd_state_bytes(sapamd_engine *e);
+int sapamd_save_state(sapamd_engine *e, void *buf, long long bytes);
+int sapamd_load_state(sapamd_engine *e, const void *buf, long long bytes);
 /* Array form of sapamd_retrieve_predicted_j_particle for n addresses: mass,
  * predicted position, velocity (and acceleration, SIXTH; zero otherwise) of
  * each, gathered on the device so that only those rows cross PCIe.  No

@@ -52,6 +52,7 @@ ENGINE_SYMBOLS = [
     "sapamd_get_grav_results", "sapamd_set_j_particles", "sapamd_last_device_ms", "sapamd_read_ngb_list", "sapamd_get_ngb_list",
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
     "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing", "sapamd_set_graphs", "sapamd_graph_count", "sapamd_retrieve_predicted_j_particles",
+    "sapamd_state_bytes", "sapamd_save_state", "sapamd_load_state",
     "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
     "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
     "sapamd_start_grav_calc_clusters",
@@ -97,6 +98,10 @@ def lib():
         L.sapamd_set_timing.argtypes = [_vp, ctypes.c_int]
         L.sapamd_set_graphs.argtypes = [_vp, ctypes.c_int]
         L.sapamd_retrieve_predicted_j_particles.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+        L.sapamd_state_bytes.argtypes = [_vp]
+        L.sapamd_state_bytes.restype = ctypes.c_longlong
+        L.sapamd_save_state.argtypes = [_vp, _vp, ctypes.c_longlong]
+        L.sapamd_load_state.argtypes = [_vp, _vp, ctypes.c_longlong]
         L.sapamd_graph_count.argtypes = [_vp]
         L.sapamd_set_split_major.argtypes = [_vp, ctypes.c_int]
         L.sapamd_set_ngb_pipeline.argtypes = [_vp, ctypes.c_int]
@@ -293,6 +298,18 @@ class Engine:
         self._check(self._L.sapamd_retrieve_predicted_j_particles(self._h, n, _pi(a), _pd(m), _pd(pos), _pd(vel),
                                                                   _pd(acc)), "retrieve_predicted_j_particles")
         return dict(mass=m, pos=pos, vel=vel, acc=acc)
+
+    def save_state(self):
+        """Checkpoint of the j-store as a uint8 array (see load_state)."""
+        nb = self._check(self._L.sapamd_state_bytes(self._h), "state_bytes")
+        buf = np.zeros(nb, np.uint8)
+        self._check(self._L.sapamd_save_state(self._h, buf.ctypes.data, nb), "save_state")
+        return buf
+
+    def load_state(self, buf):
+        """Replace the j-store by a checkpoint of an engine of the same order and precision."""
+        b = np.ascontiguousarray(buf, np.uint8)
+        self._check(self._L.sapamd_load_state(self._h, b.ctypes.data, len(b)), "load_state")
 
     def retrieve_j_particle_state(self, addr):
         m = ctypes.c_double(); idd = ctypes.c_double(); e2 = ctypes.c_double()

@@ -107,6 +107,21 @@ int sapamd_retrieve_predicted_j_particles(sapamd_engine* e, int n, const int* ad
   return e->eng.retrieve_predicted_j_particles(n, addr, mass, pos, vel, acc);
 }
 
+long long sapamd_state_bytes(sapamd_engine* e) {
+  if (!e) return -1;
+  return e->eng.state_bytes();
+}
+
+int sapamd_save_state(sapamd_engine* e, void* buf, long long bytes) {
+  if (!e) return -1;
+  return e->eng.save_state(buf, bytes);
+}
+
+int sapamd_load_state(sapamd_engine* e, const void* buf, long long bytes) {
+  if (!e) return -1;
+  return e->eng.load_state(buf, bytes);
+}
+
 int sapamd_set_graphs(sapamd_engine* e, int on) {
   if (!e) return -1;
   e->eng.set_graphs(on != 0);

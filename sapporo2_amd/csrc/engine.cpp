@@ -1176,6 +1176,130 @@ int Engine::retrieve_predicted_j_particle(int addr, double* mass, double* id, do
   return 0;
 }
 
+// Checkpoint layout: StateHeader, cluster sizes (ncl ints), then for the n
+// stored addresses: raw fields (nfr x n doubles, field-major), ids (n ints),
+// predicted records (n x record bytes), [ppos, pvel, (pacc): 3 x n doubles
+// each, mixed precision], tile id ranges (ceil(n / JTILE) int2).  Sections
+// start at 8-byte offsets.
+namespace {
+constexpr uint64_t STATE_MAGIC = 0x31534d4150415341ull;  // "ASAPAMS1"
+struct StateHeader {
+  uint64_t magic;
+  int version, order, f64, n, max_addr, nj_max, ncl, predict;
+  double t_i;
+};
+size_t pad8(size_t b) { return (b + 7) / 8 * 8; }
+}  // namespace
+
+long long Engine::state_bytes() {
+  if (!open_) return fail("state_bytes: library not open");
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (inflight_ && wait_results() != 0) return -1;
+  if (flush_updates(0) != 0) return -1;
+  const size_t n = (size_t)std::max(max_addr_ + 1, cl_total_);
+  const size_t nfr = raw_fields(order_);
+  size_t b = pad8(sizeof(StateHeader)) + pad8(sizeof(int) * cl_n_.size());
+  b += sizeof(double) * nfr * n + pad8(sizeof(int) * n) + pad8(jrec_bytes(order_, f64_) * n);
+  if (!f64_) b += sizeof(double) * 3 * n * (order_ >= ORDER_H6 ? 3 : 2);
+  b += sizeof(int2) * ((n + JTILE - 1) / JTILE);
+  return (long long)b;
+}
+
+int Engine::save_state(void* buf, long long bytes) {
+  const long long need = state_bytes();
+  if (need < 0) return -1;
+  if (!buf || bytes < need) return fail("save_state: buffer smaller than state_bytes()");
+  const int n = std::max(max_addr_ + 1, cl_total_);
+  if (ensure_store(n) != 0) return -1;
+  char* p = static_cast<char*>(buf);
+  StateHeader h{};
+  h.magic = STATE_MAGIC; h.version = 1; h.order = order_; h.f64 = f64_ ? 1 : 0; h.n = n;
+  h.max_addr = max_addr_; h.nj_max = nj_max_; h.ncl = (int)cl_n_.size(); h.predict = predict_ ? 1 : 0;
+  h.t_i = t_i_;
+  memcpy(p, &h, sizeof(h));
+  p += pad8(sizeof(h));
+  if (!cl_n_.empty()) memcpy(p, cl_n_.data(), sizeof(int) * cl_n_.size());
+  p += pad8(sizeof(int) * cl_n_.size());
+  auto d2h = [&](const void* src, size_t b) -> int {
+    if (b) HIPCHK(hipMemcpyAsync(p, src, b, hipMemcpyDeviceToHost, stream_), "state download");
+    p += pad8(b);
+    return 0;
+  };
+  const size_t cap = st_.cap;
+  for (int f = 0; f < raw_fields(order_); ++f)
+    if (d2h(st_.raw + f * cap, sizeof(double) * n) != 0) return -1;
+  if (d2h(st_.id, sizeof(int) * n) != 0) return -1;
+  if (d2h(st_.rec, jrec_bytes(order_, f64_) * n) != 0) return -1;
+  for (double* a : {st_.ppos, st_.pvel, st_.pacc})
+    if (a)
+      for (int d = 0; d < 3; ++d)
+        if (d2h(a + d * cap, sizeof(double) * n) != 0) return -1;
+  if (d2h(st_.idr, sizeof(int2) * ((n + JTILE - 1) / JTILE)) != 0) return -1;
+  HIPCHK(hipStreamSynchronize(stream_), "state download sync");
+  return 0;
+}
+
+int Engine::load_state(const void* buf, long long bytes) {
+  if (!open_) return fail("load_state: library not open");
+  if (!buf || bytes < (long long)sizeof(StateHeader)) return fail("load_state: buffer too small");
+  StateHeader h;
+  memcpy(&h, buf, sizeof(h));
+  if (h.magic != STATE_MAGIC || h.version != 1) return fail("load_state: not a state dump");
+  if (h.order != order_ || h.f64 != (f64_ ? 1 : 0))
+    return fail("load_state: dump was made with another order or precision");
+  if (h.n < 0 || h.ncl < 0 || h.max_addr >= h.n) return fail("load_state: corrupt header");
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (inflight_ && wait_results() != 0) return -1;
+  if (flush_updates(0) != 0) return -1;  // staged updates are superseded by the dump
+  const char* p = static_cast<const char*>(buf) + pad8(sizeof(h));
+  const size_t n = h.n;
+  const size_t nfr = raw_fields(order_);
+  size_t need = pad8(sizeof(h)) + pad8(sizeof(int) * (size_t)h.ncl) + sizeof(double) * nfr * n +
+                pad8(sizeof(int) * n) + pad8(jrec_bytes(order_, f64_) * n) +
+                (f64_ ? 0 : sizeof(double) * 3 * n * (order_ >= ORDER_H6 ? 3 : 2)) +
+                sizeof(int2) * ((n + JTILE - 1) / JTILE);
+  if ((size_t)bytes < need) return fail("load_state: buffer shorter than the dump it holds");
+  std::vector<int> sizes((size_t)h.ncl);
+  if (h.ncl) memcpy(sizes.data(), p, sizeof(int) * (size_t)h.ncl);
+  p += pad8(sizeof(int) * (size_t)h.ncl);
+  cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
+  if (h.ncl && set_clusters(h.ncl, sizes.data()) != 0) return -1;
+  if (cl_total_ > h.n) return fail("load_state: corrupt cluster sizes");
+  nj_max_ = std::max({h.nj_max, h.max_addr + 1, cl_total_});  // the source's logical capacity
+  if ((int)slot_of_.size() < nj_max_) slot_of_.resize(nj_max_, -1);
+  if (ensure_store(std::max(h.n, 1)) != 0) return -1;
+  // clear the whole store: addresses beyond the dump's were never set there
+  const size_t cap = st_.cap;
+  HIPCHK(hipMemsetAsync(st_.raw, 0, sizeof(double) * nfr * cap, stream_), "memset");
+  HIPCHK(hipMemsetAsync(st_.id, 0, sizeof(int) * cap, stream_), "memset");
+  HIPCHK(hipMemsetAsync(st_.rec, 0, jrec_bytes(order_, f64_) * cap, stream_), "memset");
+  HIPCHK(hipMemsetAsync(st_.idr, 0, sizeof(int2) * (cap / JTILE), stream_), "memset");
+  for (double* a : {st_.ppos, st_.pvel, st_.pacc})
+    if (a) HIPCHK(hipMemsetAsync(a, 0, sizeof(double) * 3 * cap, stream_), "memset");
+  auto h2d = [&](void* dst, size_t b) -> int {
+    if (b) HIPCHK(hipMemcpyAsync(dst, p, b, hipMemcpyHostToDevice, stream_), "state upload");
+    p += pad8(b);
+    return 0;
+  };
+  for (size_t f = 0; f < nfr; ++f)
+    if (h2d(st_.raw + f * cap, sizeof(double) * n) != 0) return -1;
+  if (h2d(st_.id, sizeof(int) * n) != 0) return -1;
+  if (h2d(st_.rec, jrec_bytes(order_, f64_) * n) != 0) return -1;
+  for (double* a : {st_.ppos, st_.pvel, st_.pacc})
+    if (a)
+      for (int d = 0; d < 3; ++d)
+        if (h2d(a + d * cap, sizeof(double) * n) != 0) return -1;
+  if (h2d(st_.idr, sizeof(int2) * ((n + JTILE - 1) / JTILE)) != 0) return -1;
+  HIPCHK(hipStreamSynchronize(stream_), "state upload sync");
+  max_addr_ = h.max_addr;
+  t_i_ = h.t_i;
+  predict_ = h.predict != 0;
+  jhost_valid_ = false;
+  have_results_ = false;
+  ngb_fetched_ = false;
+  return 0;
+}
+
 // Reference retrieve_j_particle_state (sapporohostclass.cpp:1045-1147).
 int Engine::retrieve_j_particle_state(int addr, double* mass, double* id, double* eps2, double pos[3],
                                       double vel[3], double acc[3], double jrk[3], double ppos[3],

@@ -73,6 +73,14 @@ class Engine {
   // reference, whose read-back copies every j, sapporohostclass.cpp:980-995).
   int retrieve_predicted_j_particles(int n, const int* addr, double* mass, double (*pos)[3], double (*vel)[3],
                                      double (*acc)[3]);
+  // Checkpoint of the j-store (no counterpart in the reference): every stored
+  // field, the predicted state, the block time, a pending prediction and the
+  // cluster layout, in a caller-owned host buffer.  state_bytes flushes staged
+  // updates and returns the size save_state needs; load_state needs an engine
+  // opened with the same order and precision and replaces its whole store.
+  long long state_bytes();
+  int save_state(void* buf, long long bytes);
+  int load_state(const void* buf, long long bytes);
   int retrieve_j_particle_state(int addr, double* mass, double* id, double* eps2, double pos[3],
                                 double vel[3], double acc[3], double jrk[3], double ppos[3],
                                 double pvel[3], double pacc[3]);

@@ -108,6 +108,7 @@ int Engine::open(int device, int order, int precision) {
   HIPCHK(hipEventCreate(&ev_p1_), "hipEventCreate");
   for (auto& s : stage_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(hipEventCreateWithFlags(&ev_desc_, hipEventDisableTiming), "hipEventCreate");
+  HIPCHK(hipEventCreateWithFlags(&ev_off_, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(hmalloc(&desc_h_, 1), "pinned call descriptor");
   HIPCHK(dmalloc(&desc_d_, 1), "call descriptor");
   desc_pending_ = false;
@@ -158,7 +159,7 @@ int Engine::close() {
   dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
   dfree(ens_order_d_); dfree(ens_ctr_d_); ens_order_cap_ = 0;
   icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
-  for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
+  for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_, &ev_off_}) {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
   }
@@ -759,6 +760,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
                         hipMemcpyDeviceToHost, stream_), "result download");
   HIPCHK(hipEventRecord(ev_done_, stream_), "event");
+  if (queue_list_pack(ngb_on ? ni : 0) != 0) return -1;
   inflight_ = true;
   have_results_ = true;
   ngb_fetched_ = false;
@@ -820,6 +822,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
                         hipMemcpyDeviceToHost, stream_), "result download");
   HIPCHK(hipEventRecord(ev_done_, stream_), "event");
+  if (queue_list_pack(ngb_on ? ni : 0) != 0) return -1;
   inflight_ = true;
   have_results_ = true;
   ngb_fetched_ = false;
@@ -1011,6 +1014,25 @@ int Engine::get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3],
 
 // Reference read_ngb_list (sapporohostclass.cpp:804-835): fetch the lists of
 // the last call; returns 1 if any i-particle has >= 256 neighbours.
+// Queued behind a call's result copy (after its completion event, so the
+// result wait does not include it): packing of the neighbour lists and the
+// copy of their offsets.  read_ngb_list then waits for ev_off_ and copies only
+// the packed ids.
+int Engine::queue_list_pack(int ni) {
+  if (ni <= 0) return 0;
+  PackArgs pa{};
+  pa.ngb_cnt = cnt_d_;
+  pa.ngb_list = list_d_;
+  pa.ni = ni;
+  pa.offsets = off_d_;
+  pa.packed = packed_d_;
+  if (launch_pack(stream_, pa) != 0) return fail("pack launch", hipGetLastError());
+  HIPCHK(hipMemcpyAsync(off_h_, off_d_, sizeof(int) * (size_t)(ni + 1), hipMemcpyDeviceToHost, stream_),
+         "offset download");
+  HIPCHK(hipEventRecord(ev_off_, stream_), "event");
+  return 0;
+}
+
 int Engine::read_ngb_list() {
   if (!open_ || !have_results_) return 0;
   if (wait_results() != 0) return -1;
@@ -1018,16 +1040,7 @@ int Engine::read_ngb_list() {
   const int NO = out_fields(order_);
   const int NF = part_fields(order_);
   if (ngb_on_last_) {
-    PackArgs pa{};
-    pa.ngb_cnt = cnt_d_;
-    pa.ngb_list = list_d_;
-    pa.ni = ni;
-    pa.offsets = off_d_;
-    pa.packed = packed_d_;
-    if (launch_pack(stream_, pa) != 0) return fail("pack launch", hipGetLastError());
-    HIPCHK(hipMemcpyAsync(off_h_, off_d_, sizeof(int) * (size_t)(ni + 1), hipMemcpyDeviceToHost, stream_),
-           "offset download");
-    HIPCHK(hipStreamSynchronize(stream_), "offset sync");
+    HIPCHK(hipEventSynchronize(ev_off_), "offset wait");
     const int total = off_h_[ni];
     if (total > 0)
       HIPCHK(hipMemcpyAsync(packed_h_, packed_d_, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, stream_),

@@ -115,6 +115,7 @@ class Engine {
   int ensure_store(int need);
   int ensure_partials(size_t entries);
   int flush_updates(int ni_reset);
+  int queue_list_pack(int ni);
   int upload_updates(int* nupd);
   int scatter_updates(int n, int ni_reset);
   int predict_range(int j0, int j1);
@@ -184,6 +185,7 @@ class Engine {
   sapk::CallDesc* desc_h_ = nullptr;
   sapk::CallDesc* desc_d_ = nullptr;
   hipEvent_t ev_desc_ = nullptr;
+  hipEvent_t ev_off_ = nullptr;   // neighbour-list offsets copied (queued behind each call with lists)
   bool desc_pending_ = false;
   bool graphs_ = false;
   uint64_t gen_ = 1;          // bumped whenever a buffer a captured graph uses is reallocated

@@ -274,6 +274,36 @@ def test_block_timestep_integrator_energy():
     assert finals["double"] < 1e-5 and finals["mixed"] < 1e-5, finals
 
 
+def test_hermite6_block_integrator_energy():
+    """The external block-timestep Hermite-6 integrator (predict_all, pick_up_predictor_2,
+    calc_force_on_predictors, crackle from the quintic through acc, jerk and snap at both
+    ends of a step) conserves energy through the 6th-order ABI.  The same driver linked
+    against the oracle's FP64 sums gives dE/E = 9.38e-10 on this input (162 blocks)."""
+    _say("hermite-6 block-timestep integrator")
+    exe = os.path.join(ROOT, "tests", "integrator", "hermite6_block_6th")
+    assert os.path.exists(exe), "build tests/integrator first (__graft_entry__.build)"
+    n = 1024
+    m, x, v = ics.plummer(n, seed=52)
+    path = os.path.join(os.environ.get("TMPDIR", "/tmp"), "h6b_input_%d.bin" % os.getpid())
+    with open(path, "wb") as f:
+        np.array([n], np.int32).tofile(f)
+        np.concatenate([m[:, None], x, v], axis=1).astype(np.float64).tofile(f)
+    finals = {}
+    for prec in ("double", "mixed"):
+        env = dict(os.environ, SAPPORO2_AMD_PRECISION=prec)
+        out = subprocess.run([exe, path, "0.25", "0.1", str(4.0 / n)], capture_output=True, text=True,
+                             timeout=100, env=env)
+        assert out.returncode == 0, out.stderr[-2000:]
+        blk = [l for l in out.stdout.splitlines() if l.startswith("blocks")][-1].split()
+        last = [l for l in out.stdout.splitlines() if l.startswith("final")][-1]
+        finals[prec] = abs(float(last.split()[-1]))
+        _say(f"hermite-6 block integrator {prec}: {' '.join(blk)} {last}")
+        nblocks, mean_active, min_active = int(blk[1]), float(blk[3]), int(blk[5])
+        assert nblocks > 16 and mean_active < n / 2 and min_active < 16, blk
+    os.remove(path)
+    assert finals["double"] < 1e-8 and finals["mixed"] < 1e-7, finals
+
+
 @pytest.mark.parametrize("prec", [sa.DOUBLESINGLE, sa.DOUBLE])
 def test_full_size_properties(prec):
     """N = 131072: bitwise reproducible, Newton's third law, exact NN/lists on a sample."""

@@ -25,6 +25,14 @@ constexpr int MIN_ITERS_PER_LANE = 16;
 constexpr int MIN_J_PER_SPLIT = 512;
 constexpr int FOLD = 64;   // interactions per lane between FP64 folds (kernels.hip KT)
 constexpr int IBUF_FIELDS = 10;  // xi 3, vi 3, ai 3, h2 1
+// Upload block of a single-system call, one host->device copy: the call
+// descriptor, then the i ids, then the i doubles.  The ids region is sized by
+// the power-of-two bucket of ni, so every offset depends on the bucket only
+// (the pointers a captured graph holds stay valid for its bucket).
+constexpr size_t CALL_HDR = 256;
+inline size_t call_ids_bytes(int bucket) { return ((size_t)bucket * sizeof(int) + 255) / 256 * 256; }
+constexpr size_t CALL_DOUBLES = (CALL_HDR + ((size_t)NPIPES * sizeof(int) + 255) / 256 * 256) / sizeof(double) +
+                                (size_t)IBUF_FIELDS * NPIPES;
 
 template <typename T>
 hipError_t dmalloc(T** p, size_t n) {
@@ -109,8 +117,10 @@ int Engine::open(int device, int order, int precision) {
   for (auto& s : stage_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(hipEventCreateWithFlags(&ev_desc_, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(hipEventCreateWithFlags(&ev_off_, hipEventDisableTiming), "hipEventCreate");
-  HIPCHK(hmalloc(&desc_h_, 1), "pinned call descriptor");
-  HIPCHK(dmalloc(&desc_d_, 1), "call descriptor");
+  HIPCHK(hmalloc(&call_h_, CALL_DOUBLES), "pinned call block");
+  HIPCHK(dmalloc(&call_d_, CALL_DOUBLES), "call block");
+  desc_h_ = reinterpret_cast<CallDesc*>(call_h_);
+  desc_d_ = reinterpret_cast<CallDesc*>(call_d_);
   desc_pending_ = false;
   if (const char* g = getenv("SAPPORO2_AMD_GRAPHS")) graphs_ = atoi(g) != 0;
 
@@ -137,7 +147,8 @@ int Engine::close() {
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
   drop_graphs();
-  hfree(desc_h_); dfree(desc_d_);
+  hfree(call_h_); dfree(call_d_);
+  desc_h_ = nullptr; desc_d_ = nullptr;
   if (ev_desc_) (void)hipEventDestroy(ev_desc_);
   ev_desc_ = nullptr;
   desc_pending_ = false;
@@ -592,7 +603,7 @@ int Engine::cluster_offset(int c) const {
 // doubles and ids (reference sapporohostclass.cpp:613-649); returns whether
 // any h2 > 0 (neighbour search on).
 bool Engine::pack_i(int ni, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
-                    const double* h2, double* bx, double* bv, double* ba, double* bh) {
+                    const double* h2, double* bx, double* bv, double* ba, double* bh, int* bid) {
   const size_t n3 = 3 * (size_t)ni;
   memcpy(bx, xi, sizeof(double) * n3);
   if (vi) memcpy(bv, vi, sizeof(double) * n3);
@@ -608,8 +619,8 @@ bool Engine::pack_i(int ni, const int* id, const double (*xi)[3], const double (
   } else {
     memset(bh, 0, sizeof(double) * (size_t)ni);
   }
-  if (id) memcpy(iid_h_, id, sizeof(int) * (size_t)ni);
-  else std::fill(iid_h_, iid_h_ + ni, -1);
+  if (id) memcpy(bid, id, sizeof(int) * (size_t)ni);
+  else std::fill(bid, bid + ni, -1);
   return ngb_on;
 }
 
@@ -633,7 +644,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   double* bv = bx + 3 * (size_t)ni;
   double* ba = bv + 3 * (size_t)ni;
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
-  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh);
+  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh, iid_h_);
   const size_t nd = (size_t)(bh + ni - bx);
   HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
   HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
@@ -785,26 +796,40 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   if (ensure_store(nj) != 0) return -1;
 
   // pack i-particles: [xi | vi | ai | h2] doubles and ids (reference 613-649)
-  double* bx = ibuf_h_;
+  // into the call block, once the copy that last read it has completed
+  if (desc_pending_) HIPCHK(hipEventSynchronize(ev_desc_), "call block wait");
+  desc_pending_ = false;
+  int bucket = 1;
+  while (bucket < ni) bucket <<= 1;
+  int* bid = reinterpret_cast<int*>(reinterpret_cast<char*>(call_h_) + CALL_HDR);
+  double* bx = reinterpret_cast<double*>(reinterpret_cast<char*>(call_h_) + CALL_HDR + call_ids_bytes(bucket));
+  call_iid_d_ = reinterpret_cast<int*>(reinterpret_cast<char*>(call_d_) + CALL_HDR);
+  call_ibuf_d_ = reinterpret_cast<double*>(reinterpret_cast<char*>(call_d_) + CALL_HDR + call_ids_bytes(bucket));
   double* bv = bx + 3 * (size_t)ni;
   double* ba = bv + 3 * (size_t)ni;
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
-  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh);
+  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh, bid);
   const size_t nd = (size_t)(bh + ni - bx);
-  HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
-  HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
 
   int nupd = 0;
   if (upload_updates(&nupd) != 0) return -1;
-  if (post_desc(ni, nupd, ngb_on ? ni : 0, eps2) != 0) return -1;
+  // descriptor, ids and doubles in one copy
+  desc_h_->ti = t_i_;
+  desc_h_->eps2 = eps2;
+  desc_h_->ni = ni;
+  desc_h_->nupd = nupd;
+  desc_h_->ni_reset = ngb_on ? ni : 0;
+  desc_h_->pad = 0;
+  const size_t nbytes = (size_t)((char*)(bx + nd) - (char*)call_h_);
+  HIPCHK(hipMemcpyAsync(call_d_, call_h_, nbytes, hipMemcpyHostToDevice, stream_), "call upload");
+  HIPCHK(hipEventRecord(ev_desc_, stream_), "event");
+  desc_pending_ = true;
   const bool predict = predict_;
   predict_ = false;  // reference 698
   const ForceConfig cfg = choose_config(ni, nj, ngb_on);
   // With graphs the grid covers the power-of-two bucket of ni (calls of one
   // bucket share a captured sequence); direct launches cover ni exactly.  The
   // finalize shape follows the bucket in both cases, so results agree.
-  int bucket = 1;
-  while (bucket < ni) bucket <<= 1;
   const int grid_ni = graphs_ ? std::min(bucket, NPIPES) : ni;
   if (ensure_partials((size_t)cfg.S * grid_ni) != 0) return -1;
   const int sblocks = scatter_blocks(nupd, ngb_on ? ni : 0);
@@ -855,8 +880,8 @@ int Engine::enqueue_call(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_o
   fa.rec = st_.rec;
   fa.idr = st_.idr;
   fa.nj = nj;
-  fa.ibuf = ibuf_d_;
-  fa.iid = iid_d_;
+  fa.ibuf = call_ibuf_d_;
+  fa.iid = call_iid_d_;
   fa.cd = desc_d_;
   fa.grid_ni = grid_ni;
   fa.part = part_d_;

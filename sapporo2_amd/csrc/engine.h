@@ -121,7 +121,7 @@ class Engine {
   int predict_range(int j0, int j1);
   int post_desc(int ni, int nupd, int ni_reset, double eps2);
   bool pack_i(int ni, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
-              const double* h2, double* bx, double* bv, double* ba, double* bh);
+              const double* h2, double* bx, double* bv, double* ba, double* bh, int* bid);
   int scatter_blocks(int nupd, int ni_reset) const;
   int enqueue_call(int grid_ni, int nj, const sapk::ForceConfig& cfg, bool ngb_on, bool predict,
                    int sblocks, bool rows);
@@ -182,7 +182,11 @@ class Engine {
   int ovr_P_ = 0, ovr_S_ = 0, ovr_smajor_ = 0, ovr_pipe_ = 0;
 
   // per-call descriptor (pinned host copy + device copy) and graph cache
-  sapk::CallDesc* desc_h_ = nullptr;
+  double* call_h_ = nullptr;           // pinned call block: descriptor | i ids | i doubles
+  double* call_d_ = nullptr;           // its device copy (one host->device copy per call)
+  const double* call_ibuf_d_ = nullptr;  // i doubles and ids of the current call in call_d_
+  const int* call_iid_d_ = nullptr;
+  sapk::CallDesc* desc_h_ = nullptr;   // the descriptor slot at the start of the call block
   sapk::CallDesc* desc_d_ = nullptr;
   hipEvent_t ev_desc_ = nullptr;
   hipEvent_t ev_off_ = nullptr;   // neighbour-list offsets copied (queued behind each call with lists)

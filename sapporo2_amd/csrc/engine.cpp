@@ -155,11 +155,11 @@ int Engine::close() {
   dfree(st_.raw); dfree(st_.id); dfree(st_.rec); dfree(st_.ppos); dfree(st_.pvel); dfree(st_.pacc); dfree(st_.idr);
   st_.cap = 0;
   for (auto& s : stage_) {
-    hfree(s.d); hfree(s.ai); s.cap = 0; s.busy = false;
+    hfree(s.d); s.cap = 0; s.busy = false;
     if (s.ev) (void)hipEventDestroy(s.ev);
     s.ev = nullptr;
   }
-  dfree(upd_d_); dfree(upd_ai_d_); upd_cap_d_ = 0;
+  dfree(upd_d_); upd_cap_d_ = 0;
   hfree(ibuf_h_); hfree(iid_h_); dfree(ibuf_d_); dfree(iid_d_);
   dfree(part_d_); dfree(pds2_d_); dfree(paddr_d_); part_cap_ = 0;
   dfree(out_d_); hfree(out_h_);
@@ -245,7 +245,7 @@ int Engine::ensure_store(int need) {
 }
 
 int Engine::grow_staging(int need) {
-  const int nf = raw_fields(order_);
+  const int nr = raw_fields(order_) + 1;  // row: fields, then the (address, id) pair
   for (auto& s : stage_) {
     if (s.cap >= need) continue;
     if (s.busy) {
@@ -254,17 +254,10 @@ int Engine::grow_staging(int need) {
     }
     const int nc = std::max(need, s.cap * 2);
     double* d = nullptr;
-    int* ai = nullptr;
-    HIPCHK(hmalloc(&d, (size_t)nc * nf), "pinned staging");
-    HIPCHK(hmalloc(&ai, (size_t)nc * 2), "pinned staging ids");
-    if (s.cap > 0 && &s == &stage_[cur_] && nstaged_ > 0) {
-      memcpy(d, s.d, sizeof(double) * (size_t)nstaged_ * nf);
-      memcpy(ai, s.ai, sizeof(int) * (size_t)nstaged_ * 2);
-    }
+    HIPCHK(hmalloc(&d, (size_t)nc * nr), "pinned staging");
+    if (s.cap > 0 && &s == &stage_[cur_] && nstaged_ > 0) memcpy(d, s.d, sizeof(double) * (size_t)nstaged_ * nr);
     hfree(s.d);
-    hfree(s.ai);
     s.d = d;
-    s.ai = ai;
     s.cap = nc;
   }
   return 0;
@@ -316,7 +309,7 @@ int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass,
     slot_of_[addr] = k;
   }
   const int nf = raw_fields(order_);
-  double* u = sg->d + (size_t)k * nf;
+  double* u = sg->d + (size_t)k * (nf + 1);
   u[RF_T] = tj; u[RF_DT] = dtj; u[RF_M] = mass; u[RF_EPS] = eps;
   for (int d = 0; d < 3; ++d) {
     u[RF_X + d] = x ? x[d] : 0.0;
@@ -328,8 +321,8 @@ int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass,
       u[RF_CX + d] = crk ? crk[d] : 0.0;
     }
   }
-  sg->ai[2 * k] = addr;
-  sg->ai[2 * k + 1] = id;
+  const int pair[2] = {addr, id};
+  memcpy(u + nf, pair, sizeof(pair));
   if (addr > max_addr_) max_addr_ = addr;
   jhost_valid_ = false;
   return 0;
@@ -360,27 +353,27 @@ int Engine::set_j_particles(int n, const int* addr, const int* id, const double*
 // scatter kernel is launched separately so callers can bracket device work.
 int Engine::upload_updates(int* nupd) {
   const int n = nstaged_;
-  const int nf = raw_fields(order_);
+  const int nr = raw_fields(order_) + 1;
   *nupd = n;
   if (n <= 0) return 0;
   if (ensure_store(max_addr_ + 1) != 0) return -1;
   if (upd_cap_d_ < n) {
     dfree(upd_d_);
-    dfree(upd_ai_d_);
     const int nc = std::max(n, upd_cap_d_ * 2);
-    HIPCHK(dmalloc(&upd_d_, (size_t)nc * nf), "device staging");
-    HIPCHK(dmalloc(&upd_ai_d_, (size_t)nc * 2), "device staging ids");
+    HIPCHK(dmalloc(&upd_d_, (size_t)nc * nr), "device staging");
     upd_cap_d_ = nc;
     ++gen_;
   }
   Stage& sg = stage_[cur_];
-  HIPCHK(hipMemcpyAsync(upd_d_, sg.d, sizeof(double) * (size_t)n * nf, hipMemcpyHostToDevice, stream_),
-         "update upload");
-  HIPCHK(hipMemcpyAsync(upd_ai_d_, sg.ai, sizeof(int) * (size_t)n * 2, hipMemcpyHostToDevice, stream_),
+  HIPCHK(hipMemcpyAsync(upd_d_, sg.d, sizeof(double) * (size_t)n * nr, hipMemcpyHostToDevice, stream_),
          "update upload");
   HIPCHK(hipEventRecord(sg.ev, stream_), "event");
   sg.busy = true;
-  for (int k = 0; k < n; ++k) slot_of_[sg.ai[2 * k]] = -1;
+  for (int k = 0; k < n; ++k) {
+    int pair[2];
+    memcpy(pair, sg.d + (size_t)k * nr + nr - 1, sizeof(pair));
+    slot_of_[pair[0]] = -1;
+  }
   nstaged_ = 0;
   cur_ ^= 1;
   return 0;
@@ -400,7 +393,6 @@ int Engine::scatter_updates(int n, int ni_reset) {
   ScatterArgs a{};
   a.st = st_;
   a.upd = upd_d_;
-  a.upd_ai = upd_ai_d_;
   a.nf = raw_fields(order_);
   a.ngb_cnt = cnt_d_;
   a.cd = desc_d_;
@@ -868,8 +860,7 @@ int Engine::enqueue_call(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_o
     ScatterArgs a{};
     a.st = st_;
     a.upd = upd_d_;
-    a.upd_ai = upd_ai_d_;
-    a.nf = raw_fields(order_);
+      a.nf = raw_fields(order_);
     a.ngb_cnt = cnt_d_;
     a.cd = desc_d_;
     a.grid_blocks = sblocks;

@@ -146,8 +146,7 @@ class Engine {
 
   // staged updates (pinned), double-buffered
   struct Stage {
-    double* d = nullptr;      // cap x nf doubles
-    int* ai = nullptr;        // cap x 2 ints
+    double* d = nullptr;      // cap rows of nf doubles + one (address, id) int pair
     int cap = 0;
     hipEvent_t ev = nullptr;  // completion of the copy that last read it
     bool busy = false;
@@ -156,7 +155,6 @@ class Engine {
   int cur_ = 0, nstaged_ = 0;
   std::vector<int> slot_of_;  // address -> staged slot or -1
   double* upd_d_ = nullptr;
-  int* upd_ai_d_ = nullptr;
   int upd_cap_d_ = 0;
 
   // prediction state

@@ -89,8 +89,7 @@ struct CallDesc {
 // cd->nupd and cd->ni_reset).
 struct ScatterArgs {
   StoreView st;
-  const double* upd;     // nupd x nf doubles (fields 0..nf-1 of RawField)
-  const int* upd_ai;     // nupd x 2 ints: address, id
+  const double* upd;     // nupd rows of nf + 1 doubles: fields 0..nf-1 of RawField, then (address, id) as an int2
   int nf;
   int* ngb_cnt;          // zeroed for [0, ni_reset)
   const CallDesc* cd;

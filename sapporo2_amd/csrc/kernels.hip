@@ -74,9 +74,10 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   for (int k = tid; k < ni_reset; k += stride) a.ngb_cnt[k] = 0;
   const size_t cap = (size_t)a.st.cap;
   for (int k = tid; k < nupd; k += stride) {
-    const double* u = a.upd + (size_t)k * a.nf;
-    const int addr = a.upd_ai[2 * k];
-    const int id = a.upd_ai[2 * k + 1];
+    const double* u = a.upd + (size_t)k * (a.nf + 1);
+    const int2 pair = *reinterpret_cast<const int2*>(u + a.nf);  // address, id
+    const int addr = pair.x;
+    const int id = pair.y;
     if (addr < 0 || addr >= a.st.cap) continue;  // host guarantees range
     for (int f = 0; f < a.nf; ++f) a.st.raw[(size_t)f * cap + addr] = u[f];
     a.st.id[addr] = id;

@@ -12,9 +12,17 @@
  *   inside; pot is negative (-sum m/r); inn is the nearest neighbour's id;
  *   neighbour lists are ids sorted ascending; eps2 is global per call.
  * Precision: the reference opens FOURTH/DOUBLESINGLE (sapporoG6lib.cpp:34).
- *   Here the default is the mixed-precision kernel (FP64 relative positions,
- *   FP32 pair terms, FP64-folded accumulation); SAPPORO2_AMD_PRECISION=double
- *   selects the all-FP64 kernel.  Both ignore cluster_id, as the reference.
+ *   Here the default is the mixed-precision kernel (positions as double-single
+ *   pairs, relative positions by the double-single difference
+ *   (xj.hi - xi.hi) + (xj.lo - xi.lo) in FP32, FP32 pair terms, per-tile FP32
+ *   sums folded into FP64); SAPPORO2_AMD_PRECISION=double selects the all-FP64
+ *   kernel.  Both ignore cluster_id, as the reference.
+ *
+ * The second part of this header is the caller side of the reference header
+ * (testCodes/src/g6lib.h:52-90): by-value wrappers, the position-only
+ * multisend helper and the no-op stubs.  The reference defines them as plain
+ * (external) functions in its header; here they are static inline, so the
+ * header can be included by several translation units of one program.
  */
 #ifndef SAPPORO2_AMD_G6LIB_H
 #define SAPPORO2_AMD_G6LIB_H
@@ -76,6 +84,49 @@ void get_j_part_data(int addr, int nj, double *pos, double *vel, double *acc, do
 void get_j_part_data(int addr, int nj, double *pos, double *vel, double *acc, double *jrk,
                      double *ppos, double *pvel, double *mass, double *eps2, int *id);
 #endif
+
+/* ---- caller side: testCodes/src/g6lib.h:52-62, by-value wrappers ---- */
+static inline int g6_close(int clusterid) { return g6_close_(&clusterid); }
+static inline int g6_open(int clusterid) { return g6_open_(&clusterid); }
+static inline int g6_npipes(void) { return g6_npipes_(); }
+static inline void g6_set_ti(int cluster_id, double ti) { g6_set_ti_(&cluster_id, &ti); }
+static inline int g6_reset(int cluster_id) { return g6_reset_(&cluster_id); }
+static inline int g6_reset_fofpga(int cluster_id) { return g6_reset_fofpga_(&cluster_id); }
+static inline int g6_set_j_particle(int clusterid, int address, int index, double tj, double dtj, double mass,
+                                    double k18[3], double j6[3], double a2[3], double v[3], double x[3]) {
+  return g6_set_j_particle_(&clusterid, &address, &index, &tj, &dtj, &mass, k18, j6, a2, v, x);
+}
+static inline void g6calc_firsthalf(int cluster_id, int nj, int ni, int index[], double xi[][3], double vi[][3],
+                                    double aold[][3], double j6old[][3], double phiold[], double eps2,
+                                    double h2[]) {
+  g6calc_firsthalf_(&cluster_id, &nj, &ni, index, xi, vi, aold, j6old, phiold, &eps2, h2);
+}
+static inline int g6calc_lasthalf(int cluster_id, int nj, int ni, int index[], double xi[][3], double vi[][3],
+                                  double eps2, double h2[], double acc[][3], double jerk[][3], double pot[]) {
+  return g6calc_lasthalf_(&cluster_id, &nj, &ni, index, xi, vi, &eps2, h2, acc, jerk, pot);
+}
+static inline void g6_set_tunit(double newtunit) { (void)g6_set_tunit_(&newtunit); }
+static inline void g6_set_xunit(double newxunit) { (void)g6_set_xunit_(&newxunit); }
+
+/* testCodes/src/g6lib.h:67-84: stores particle `address` with its mass and
+ * position only (time, velocity, acceleration and jerk zero). */
+static inline int g6_set_j_particle_multisend_mxfast_(int *clusterid, int *nclusters, int *address, int *index,
+                                                      double *mass, double x[3]) {
+  double zero3a[3] = {0, 0, 0}, zero3b[3] = {0, 0, 0}, zero3c[3] = {0, 0, 0}, zero3d[3] = {0, 0, 0};
+  double tj = 0, dtj = 0;
+  (void)nclusters;
+  return g6_set_j_particle_(clusterid, address, index, &tj, &dtj, mass, zero3a, zero3b, zero3c, zero3d, x);
+}
+
+/* testCodes/src/g6lib.h:86-90: hardware-management calls of GRAPE-6 boards,
+ * no-ops here (the reference's bodies return nothing; these return 0). */
+static inline int g6_reinitialize(int clusterid) { (void)clusterid; return 0; }
+static inline int g6_print_chip_status(int clusterid) { (void)clusterid; return 0; }
+static inline int g6_flush_jp_buffer_and_multisend(int clusterid, int something) {
+  (void)clusterid; (void)something; return 0;
+}
+static inline int g6_initialize_jp_buffer(int clusterid, int list_max) { (void)clusterid; (void)list_max; return 0; }
+static inline int g6_set_overflow_flag_test_mode_(int *clusterid) { (void)clusterid; return 0; }
 
 #ifdef __cplusplus
 }

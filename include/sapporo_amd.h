@@ -136,6 +136,11 @@ int sapamd_load_tuning(sapamd_engine *e, const char *path);
 int sapamd_tuning_entries(sapamd_engine *e);
 int sapamd_num_cus(sapamd_engine *e);
 int sapamd_synchronize(sapamd_engine *e);
+/* The engine behind the GRAPE-6 ABI's global instance (include/g6lib.h), so
+ * that a g6 caller can switch on timing or read launch shapes with the calls
+ * above.  No reference counterpart (the reference's instance is file-static,
+ * sapporoG6lib.cpp:3).  Valid for the life of the process; do not destroy. */
+sapamd_engine *sapamd_g6_engine(void);
 const char *sapamd_last_error(sapamd_engine *e);
 
 #ifdef __cplusplus

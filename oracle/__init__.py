@@ -33,11 +33,32 @@ _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
-def lib():
-    """Load liboracle.so, building it with make if it is missing."""
+def build_native(outdir):
+    """Compile oracle.c for this host's CPU (``-O3 -march=native``, no FP contraction)
+    into ``outdir`` and load it in place of the portable build.  Same arithmetic: the
+    restatement has no reductions the compiler may reorder without -ffast-math, and
+    fma/sqrt are correctly rounded either way.  Returns the compile command, or None
+    (the portable x86-64-v3 build stays loaded) when the compiler is unavailable."""
+    import subprocess
+    global _LIB
+    out = os.path.join(outdir, "liboracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+           "-shared", "-o", out, os.path.join(_HERE, "oracle.c"), "-lm"]
+    try:
+        subprocess.check_call(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    except (OSError, subprocess.CalledProcessError):
+        return None
+    _LIB = None
+    lib(out)
+    return " ".join(cmd[:6] + ["oracle.c"])
+
+
+def lib(path=None):
+    """Load liboracle.so (or the library at ``path``), building it with make if it is missing."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        if path is None:
+            path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             import subprocess
             subprocess.check_call(["make", "-s", "-C", _HERE])

@@ -5,10 +5,6 @@
 
 #include "engine.h"
 
-struct sapamd_engine {
-  sap::Engine eng;
-};
-
 using V3 = const double (*)[3];
 using W3 = double (*)[3];
 

@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <dlfcn.h>
+#include <immintrin.h>
 
 #include <algorithm>
 #include <cmath>
@@ -25,6 +26,9 @@ constexpr int MIN_ITERS_PER_LANE = 16;
 constexpr int MIN_J_PER_SPLIT = 512;
 constexpr int FOLD = 64;   // interactions per lane between FP64 folds (kernels.hip KT)
 constexpr int IBUF_FIELDS = 10;  // xi 3, vi 3, ai 3, h2 1
+// Staged j-update rows are copied to the device as soon as this many are
+// pending, while the caller is still staging (the copy overlaps its loop).
+constexpr int EARLY_FLUSH_ROWS = 4096;
 // Upload block of a single-system call, one host->device copy: the call
 // descriptor, then the i ids, then the i doubles.  The ids region is sized by
 // the power-of-two bucket of ni, so every offset depends on the bucket only
@@ -45,6 +49,17 @@ hipError_t hmalloc(T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
   return hipHostMalloc(reinterpret_cast<void**>(p), n * sizeof(T), hipHostMallocDefault);
+}
+// Pinned host memory mapped into the device's address space, coherent: the
+// kernels write results into it directly (host pointer h, device alias d).
+template <typename T>
+hipError_t hmap(T** h, T** d, size_t n) {
+  *h = *d = nullptr;
+  if (n == 0) n = 1;
+  const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(h), n * sizeof(T),
+                                     hipHostMallocMapped | hipHostMallocCoherent);
+  if (e != hipSuccess) return e;
+  return hipHostGetDevicePointer(reinterpret_cast<void**>(d), *h, 0);
 }
 template <typename T>
 void dfree(T*& p) {
@@ -116,12 +131,14 @@ int Engine::open(int device, int order, int precision) {
   HIPCHK(hipEventCreate(&ev_p1_), "hipEventCreate");
   for (auto& s : stage_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(hipEventCreateWithFlags(&ev_desc_, hipEventDisableTiming), "hipEventCreate");
-  HIPCHK(hipEventCreateWithFlags(&ev_off_, hipEventDisableTiming), "hipEventCreate");
+  HIPCHK(dmalloc(&pack_ctr_d_, 1), "pack counter");
   HIPCHK(hmalloc(&call_h_, CALL_DOUBLES), "pinned call block");
   HIPCHK(dmalloc(&call_d_, CALL_DOUBLES), "call block");
   desc_h_ = reinterpret_cast<CallDesc*>(call_h_);
   desc_d_ = reinterpret_cast<CallDesc*>(call_d_);
   desc_pending_ = false;
+  rec_seq_ = done_seq_ = done_rec_seq_ = desc_seq_ = 0;
+  flushed_ = 0;
   if (const char* g = getenv("SAPPORO2_AMD_GRAPHS")) graphs_ = atoi(g) != 0;
 
   icap_ = 0;
@@ -162,15 +179,18 @@ int Engine::close() {
   dfree(upd_d_); upd_cap_d_ = 0;
   hfree(ibuf_h_); hfree(iid_h_); dfree(ibuf_d_); dfree(iid_d_);
   dfree(part_d_); dfree(pds2_d_); dfree(paddr_d_); part_cap_ = 0;
-  dfree(out_d_); hfree(out_h_);
-  dfree(cnt_d_); dfree(list_d_); dfree(off_d_); dfree(packed_d_);
-  hfree(off_h_); hfree(packed_h_);
+  hfree(res_h_); res_hd_ = nullptr; res_stride_ = 0;
+  hfree(loff_h_); loff_hd_ = nullptr;
+  hfree(packed_h_); packed_hd_ = nullptr; packed_cap_ = 0;
+  dfree(pack_ctr_d_);
+  dfree(cnt_d_); dfree(list_d_); list_cap_ = 0;
+  slot_lists_h_.clear();
   dfree(gather_d_); gather_cap_ = 0;
   dfree(gaddr_d_); gaddr_cap_ = 0;
   dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
   dfree(ens_order_d_); dfree(ens_ctr_d_); ens_order_cap_ = 0;
   icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
-  for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_, &ev_off_}) {
+  for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
   }
@@ -248,10 +268,8 @@ int Engine::grow_staging(int need) {
   const int nr = raw_fields(order_) + 1;  // row: fields, then the (address, id) pair
   for (auto& s : stage_) {
     if (s.cap >= need) continue;
-    if (s.busy) {
-      HIPCHK(hipEventSynchronize(s.ev), "staging wait");
-      s.busy = false;
-    }
+    if (s.busy && s.seq > done_seq_) HIPCHK(hipEventSynchronize(s.ev), "staging wait");
+    s.busy = false;
     const int nc = std::max(need, s.cap * 2);
     double* d = nullptr;
     HIPCHK(hmalloc(&d, (size_t)nc * nr), "pinned staging");
@@ -295,8 +313,11 @@ int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass,
   }
   if ((int)slot_of_.size() < nj_max_) slot_of_.resize(nj_max_, -1);
   Stage* sg = &stage_[cur_];
-  if (sg->busy) {
-    HIPCHK(hipEventSynchronize(sg->ev), "staging wait");
+  // a stage still read by the copy of an earlier flush is waited for before its
+  // first row is rewritten; copies of this flush's own rows (early flushes)
+  // need no wait: a rewritten row is copied again behind them
+  if (nstaged_ == 0 && sg->busy) {
+    if (sg->seq > done_seq_) HIPCHK(hipEventSynchronize(sg->ev), "staging wait");
     sg->busy = false;
   }
   int k = slot_of_[addr];
@@ -307,6 +328,8 @@ int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass,
     }
     k = nstaged_++;
     slot_of_[addr] = k;
+  } else if (k < flushed_) {
+    flushed_ = k;  // already copied: copy again from this row
   }
   const int nf = raw_fields(order_);
   double* u = sg->d + (size_t)k * (nf + 1);
@@ -325,6 +348,34 @@ int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass,
   memcpy(u + nf, pair, sizeof(pair));
   if (addr > max_addr_) max_addr_ = addr;
   jhost_valid_ = false;
+  if (nstaged_ - flushed_ >= EARLY_FLUSH_ROWS) return copy_staged();
+  return 0;
+}
+
+// Copies the staged rows [flushed_, nstaged_) of the current stage to the
+// device staging buffer (asynchronously, behind earlier work on the stream).
+// The device buffer only grows; after a regrowth every staged row is copied.
+int Engine::copy_staged() {
+  const int n = nstaged_;
+  if (n <= flushed_) return 0;
+  const int nr = raw_fields(order_) + 1;
+  if (upd_cap_d_ < n) {
+    if (stream_) HIPCHK(hipStreamSynchronize(stream_), "staging grow sync");
+    dfree(upd_d_);
+    const int nc = std::max(n, upd_cap_d_ * 2);
+    HIPCHK(dmalloc(&upd_d_, (size_t)nc * nr), "device staging");
+    upd_cap_d_ = nc;
+    flushed_ = 0;
+    ++gen_;
+  }
+  Stage& sg = stage_[cur_];
+  const size_t o = (size_t)flushed_ * nr;
+  HIPCHK(hipMemcpyAsync(upd_d_ + o, sg.d + o, sizeof(double) * ((size_t)n * nr - o), hipMemcpyHostToDevice,
+                        stream_), "update upload");
+  HIPCHK(hipEventRecord(sg.ev, stream_), "event");
+  sg.busy = true;
+  sg.seq = ++rec_seq_;
+  flushed_ = n;
   return 0;
 }
 
@@ -357,24 +408,15 @@ int Engine::upload_updates(int* nupd) {
   *nupd = n;
   if (n <= 0) return 0;
   if (ensure_store(max_addr_ + 1) != 0) return -1;
-  if (upd_cap_d_ < n) {
-    dfree(upd_d_);
-    const int nc = std::max(n, upd_cap_d_ * 2);
-    HIPCHK(dmalloc(&upd_d_, (size_t)nc * nr), "device staging");
-    upd_cap_d_ = nc;
-    ++gen_;
-  }
+  if (copy_staged() != 0) return -1;
   Stage& sg = stage_[cur_];
-  HIPCHK(hipMemcpyAsync(upd_d_, sg.d, sizeof(double) * (size_t)n * nr, hipMemcpyHostToDevice, stream_),
-         "update upload");
-  HIPCHK(hipEventRecord(sg.ev, stream_), "event");
-  sg.busy = true;
   for (int k = 0; k < n; ++k) {
     int pair[2];
     memcpy(pair, sg.d + (size_t)k * nr + nr - 1, sizeof(pair));
     slot_of_[pair[0]] = -1;
   }
   nstaged_ = 0;
+  flushed_ = 0;
   cur_ ^= 1;
   return 0;
 }
@@ -389,14 +431,16 @@ int Engine::scatter_blocks(int n, int ni_reset) const {
   return nb;
 }
 
-int Engine::scatter_updates(int n, int ni_reset) {
+int Engine::scatter_updates(int n, int ni_reset, int pred_j1) {
   ScatterArgs a{};
   a.st = st_;
   a.upd = upd_d_;
   a.nf = raw_fields(order_);
   a.ngb_cnt = cnt_d_;
+  a.pack_ctr = pack_ctr_d_;
   a.cd = desc_d_;
   a.grid_blocks = scatter_blocks(n, ni_reset);
+  a.pred_j1 = pred_j1;
   if (a.grid_blocks == 0) return 0;
   if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
   return 0;
@@ -405,7 +449,7 @@ int Engine::scatter_updates(int n, int ni_reset) {
 // Writes the call descriptor (pinned host copy) and queues its upload.  The
 // previous upload has to have read the host copy before it is rewritten.
 int Engine::post_desc(int ni, int nupd, int ni_reset, double eps2) {
-  if (desc_pending_) HIPCHK(hipEventSynchronize(ev_desc_), "descriptor wait");
+  if (desc_pending_ && desc_seq_ > done_seq_) HIPCHK(hipEventSynchronize(ev_desc_), "descriptor wait");
   desc_h_->ti = t_i_;
   desc_h_->eps2 = eps2;
   desc_h_->ni = ni;
@@ -415,14 +459,15 @@ int Engine::post_desc(int ni, int nupd, int ni_reset, double eps2) {
   HIPCHK(hipMemcpyAsync(desc_d_, desc_h_, sizeof(CallDesc), hipMemcpyHostToDevice, stream_), "descriptor upload");
   HIPCHK(hipEventRecord(ev_desc_, stream_), "event");
   desc_pending_ = true;
+  desc_seq_ = ++rec_seq_;
   return 0;
 }
 
-int Engine::flush_updates(int ni_reset) {
+int Engine::flush_updates(int ni_reset, int pred_j1) {
   int n = 0;
   if (upload_updates(&n) != 0) return -1;
   if (post_desc(0, n, ni_reset, 0.0) != 0) return -1;
-  return scatter_updates(n, ni_reset);
+  return scatter_updates(n, ni_reset, pred_j1);
 }
 
 int Engine::predict_range(int j0, int j1) {
@@ -531,34 +576,78 @@ ForceConfig Engine::choose_config(int ni, int nj, bool ngb) const {
   return ForceConfig{P, (int)S, (int)jchunk, ovr_smajor_, ovr_pipe_};
 }
 
-// i-particle-sized buffers (inputs, result rows, neighbour lists) for up to
-// `need` i-particles per call.
+// i-particle-sized buffers (ensemble inputs, host-mapped result rows and list
+// offsets, neighbour counts) for up to `need` i-particles per call.  The
+// neighbour-list slots and the packed host lists are sized separately, only
+// for calls with lists (ensure_lists, ensure_packed).
 int Engine::ensure_icap(int need) {
   if (need <= icap_) return 0;
   if (stream_) HIPCHK(hipStreamSynchronize(stream_), "i-buffer grow sync");
   inflight_ = false;
-  hfree(ibuf_h_); hfree(iid_h_); dfree(ibuf_d_); dfree(iid_d_); dfree(out_d_); hfree(out_h_);
-  dfree(cnt_d_); dfree(list_d_); dfree(off_d_); dfree(packed_d_); hfree(off_h_); hfree(packed_h_);
-  dfree(idesc_d_);
+  hfree(ibuf_h_); hfree(iid_h_); dfree(ibuf_d_); dfree(iid_d_);
+  hfree(res_h_); hfree(loff_h_); dfree(cnt_d_); dfree(idesc_d_);
   const size_t nI = (size_t)need;
   HIPCHK(hmalloc(&ibuf_h_, IBUF_FIELDS * nI), "pinned i buffer");
   HIPCHK(hmalloc(&iid_h_, nI), "pinned i ids");
   HIPCHK(dmalloc(&ibuf_d_, IBUF_FIELDS * nI), "i buffer");
   HIPCHK(dmalloc(&iid_d_, nI), "i ids");
-  HIPCHK(dmalloc(&out_d_, nI * out_fields(order_)), "result buffer");
-  HIPCHK(hmalloc(&out_h_, nI * out_fields(order_)), "pinned results");
+  HIPCHK(hmap(&res_h_, &res_hd_, nI * out_fields(order_)), "host-mapped results");
+  res_stride_ = (long)nI;
+  HIPCHK(hmap(&loff_h_, &loff_hd_, nI), "host-mapped list offsets");
   HIPCHK(dmalloc(&cnt_d_, nI), "neighbour counts");
-  HIPCHK(dmalloc(&list_d_, nI * NGB_CAP), "neighbour lists");
-  HIPCHK(dmalloc(&off_d_, nI + 1), "neighbour offsets");
-  HIPCHK(dmalloc(&packed_d_, nI * NGB_CAP), "packed lists");
-  HIPCHK(hmalloc(&off_h_, nI + 1), "pinned offsets");
-  HIPCHK(hmalloc(&packed_h_, nI * NGB_CAP), "pinned packed lists");
   HIPCHK(dmalloc(&idesc_d_, nI), "ensemble i descriptors");
   HIPCHK(hipMemsetAsync(cnt_d_, 0, nI * sizeof(int), stream_), "memset");
   icap_ = need;
   have_results_ = false;
   ++gen_;
   return 0;
+}
+
+// Device neighbour-list slots (NGB_CAP per i-particle) for `need` i-particles.
+int Engine::ensure_lists(int need) {
+  if (need <= list_cap_) return 0;
+  if (stream_) HIPCHK(hipStreamSynchronize(stream_), "list grow sync");
+  inflight_ = false;
+  dfree(list_d_);
+  HIPCHK(dmalloc(&list_d_, (size_t)need * NGB_CAP), "neighbour lists");
+  list_cap_ = need;
+  have_results_ = false;
+  ++gen_;
+  return 0;
+}
+
+// Host-mapped packed lists of at least `need` ids.  A call whose lists do not
+// fit is still correct: read_ngb_list then copies the device slots back.
+int Engine::ensure_packed(long need) {
+  if (need <= packed_cap_) return 0;
+  if (stream_) HIPCHK(hipStreamSynchronize(stream_), "packed-list grow sync");
+  inflight_ = false;
+  hfree(packed_h_);
+  HIPCHK(hmap(&packed_h_, &packed_hd_, (size_t)need), "host-mapped packed lists");
+  packed_cap_ = need;
+  have_results_ = false;
+  ++gen_;
+  return 0;
+}
+
+FinalizeArgs Engine::finalize_args(int grid_ni, bool ngb_on) const {
+  FinalizeArgs fz{};
+  fz.part = part_d_;
+  fz.part_ds2 = pds2_d_;
+  fz.part_addr = paddr_d_;
+  fz.cd = desc_d_;
+  fz.grid_ni = grid_ni;
+  fz.rec = st_.rec;
+  fz.ngb_cnt = cnt_d_;
+  fz.ngb_list = list_d_;
+  fz.out = res_hd_;
+  fz.ostride = res_stride_;
+  fz.list_off = loff_hd_;
+  fz.packed = packed_hd_;
+  fz.packed_cap = packed_cap_;
+  fz.pack_ctr = pack_ctr_d_;
+  fz.ngb_on = ngb_on ? 1 : 0;
+  return fz;
 }
 
 // Ensemble layout: cluster c owns the addresses [offset(c), offset(c) + nj_c),
@@ -631,6 +720,15 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   HIPCHK(hipSetDevice(device_), "hipSetDevice");
   if (inflight_ && wait_results() != 0) return -1;
   if (ensure_icap(ni) != 0) return -1;
+  bool has_h2 = false;
+  if (h2)
+    for (int i = 0; i < ni && !has_h2; ++i) has_h2 = h2[i] > 0.0;
+  if (has_h2) {
+    // packed host lists sized for a mean of 64 neighbours (larger totals fall
+    // back to reading the device slots)
+    if (ensure_lists(ni) != 0) return -1;
+    if (ensure_packed(std::max<long>((long)NPIPES * NGB_CAP, 64L * ni)) != 0) return -1;
+  }
 
   double* bx = ibuf_h_;
   double* bv = bx + 3 * (size_t)ni;
@@ -716,7 +814,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   if (upload_updates(&nupd) != 0) return -1;
   if (post_desc(ni, nupd, ngb_on ? ni : 0, 0.0) != 0) return -1;
   if (timing_) HIPCHK(hipEventRecord(ev_d0_, stream_), "event");
-  if (scatter_updates(nupd, ngb_on ? ni : 0) != 0) return -1;
+  if (scatter_updates(nupd, ngb_on ? ni : 0, predict_ ? cl_total_ : 0) != 0) return -1;
   if (predict_) {
     if (predict_range(0, cl_total_) != 0) return -1;
   } else {
@@ -744,14 +842,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ea.ngb_cnt = cnt_d_;
   ea.ngb_list = list_d_;
   ea.ngb_on = ngb_on ? 1 : 0;
-  FinalizeArgs fz{};
-  fz.cd = desc_d_;
-  fz.grid_ni = ni;
-  fz.rec = st_.rec;
-  fz.ngb_cnt = cnt_d_;
-  fz.ngb_list = list_d_;
-  fz.out = out_d_;
-  fz.ngb_on = ea.ngb_on;
+  const FinalizeArgs fz = finalize_args(ni, ngb_on);
   if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
   if (launch_force_ens(stream_, order_, f64_, ea, fz, n64, (long)cus_ * 4 * 8) != 0)
     return fail("ensemble force launch", hipGetLastError());
@@ -760,13 +851,12 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
     HIPCHK(hipEventRecord(ev_d1_, stream_), "event");
   }
   timed_force_ = timing_;
-  HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
-                        hipMemcpyDeviceToHost, stream_), "result download");
   HIPCHK(hipEventRecord(ev_done_, stream_), "event");
-  if (queue_list_pack(ngb_on ? ni : 0) != 0) return -1;
+  done_rec_seq_ = ++rec_seq_;
   inflight_ = true;
   have_results_ = true;
   ngb_fetched_ = false;
+  lists_in_slots_ = false;
   ngb_on_last_ = ngb_on;
   ni_last_ = ni;
   nj_last_ = cl_total_;
@@ -789,7 +879,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
 
   // pack i-particles: [xi | vi | ai | h2] doubles and ids (reference 613-649)
   // into the call block, once the copy that last read it has completed
-  if (desc_pending_) HIPCHK(hipEventSynchronize(ev_desc_), "call block wait");
+  if (desc_pending_ && desc_seq_ > done_seq_) HIPCHK(hipEventSynchronize(ev_desc_), "call block wait");
   desc_pending_ = false;
   int bucket = 1;
   while (bucket < ni) bucket <<= 1;
@@ -802,6 +892,10 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
   const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh, bid);
   const size_t nd = (size_t)(bh + ni - bx);
+  if (ngb_on) {
+    if (ensure_lists(NPIPES) != 0) return -1;
+    if (ensure_packed((long)NPIPES * NGB_CAP) != 0) return -1;
+  }
 
   int nupd = 0;
   if (upload_updates(&nupd) != 0) return -1;
@@ -816,6 +910,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   HIPCHK(hipMemcpyAsync(call_d_, call_h_, nbytes, hipMemcpyHostToDevice, stream_), "call upload");
   HIPCHK(hipEventRecord(ev_desc_, stream_), "event");
   desc_pending_ = true;
+  desc_seq_ = ++rec_seq_;
   const bool predict = predict_;
   predict_ = false;  // reference 698
   const ForceConfig cfg = choose_config(ni, nj, ngb_on);
@@ -836,13 +931,12 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
     timed_force_ = timing_;
   }
   if (timing_) HIPCHK(hipEventRecord(ev_d1_, stream_), "event");
-  HIPCHK(hipMemcpyAsync(out_h_, out_d_, sizeof(double) * (size_t)ni * out_fields(order_),
-                        hipMemcpyDeviceToHost, stream_), "result download");
   HIPCHK(hipEventRecord(ev_done_, stream_), "event");
-  if (queue_list_pack(ngb_on ? ni : 0) != 0) return -1;
+  done_rec_seq_ = ++rec_seq_;
   inflight_ = true;
   have_results_ = true;
   ngb_fetched_ = false;
+  lists_in_slots_ = false;
   ngb_on_last_ = ngb_on;
   ni_last_ = ni;
   nj_last_ = nj;
@@ -862,8 +956,10 @@ int Engine::enqueue_call(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_o
     a.upd = upd_d_;
       a.nf = raw_fields(order_);
     a.ngb_cnt = cnt_d_;
+    a.pack_ctr = pack_ctr_d_;
     a.cd = desc_d_;
     a.grid_blocks = sblocks;
+    a.pred_j1 = predict ? nj : 0;
     if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
   }
   if (predict && predict_range(0, nj) != 0) return -1;
@@ -885,19 +981,9 @@ int Engine::enqueue_call(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_o
   if (ev) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
   if (launch_force(stream_, order_, f64_, fa, cfg) != 0) return fail("force launch", hipGetLastError());
   if (ev) HIPCHK(hipEventRecord(ev_f1_, stream_), "event");
-  FinalizeArgs fz{};
-  fz.part = part_d_;
-  fz.part_ds2 = pds2_d_;
-  fz.part_addr = paddr_d_;
+  FinalizeArgs fz = finalize_args(grid_ni, ngb_on);
   fz.S = partial_rows(cfg);
-  fz.cd = desc_d_;
-  fz.grid_ni = grid_ni;
   fz.rows = rows ? 1 : 0;
-  fz.rec = st_.rec;
-  fz.ngb_cnt = cnt_d_;
-  fz.ngb_list = list_d_;
-  fz.out = out_d_;
-  fz.ngb_on = fa.ngb_on;
   if (launch_finalize(stream_, order_, f64_, fz) != 0) return fail("finalize launch", hipGetLastError());
   return 0;
 }
@@ -964,11 +1050,19 @@ void Engine::drop_graphs() {
   graph_cache_.clear();
 }
 
+// Waits for the last call's completion event by polling it (the caller is
+// blocked anyway; polling wakes within a microsecond or two, where a blocking
+// wait costs tens).  Everything recorded before that event is then complete.
 int Engine::wait_results() {
   if (!inflight_) return 0;
-  HIPCHK(hipEventSynchronize(ev_done_), "result wait");
+  hipError_t e;
+  while ((e = hipEventQuery(ev_done_)) == hipErrorNotReady) {
+    for (int k = 0; k < 32; ++k) _mm_pause();
+  }
+  if (e != hipSuccess) return fail("result wait", e);
   HIPCHK(hipGetLastError(), "device error");
   inflight_ = false;
+  done_seq_ = done_rec_seq_;
   return 0;
 }
 
@@ -1005,70 +1099,59 @@ int Engine::get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3],
   if (ni == 0 || nj == 0) return 0;  // reference 718
   if (!have_results_ || ni > ni_last_) return fail("get_grav_results: no matching firsthalf");
   if (wait_results() != 0) return -1;
-  const int NO = out_fields(order_);
   const int NF = part_fields(order_);
-  for (int i = 0; i < ni; ++i) {
-    const double* o = out_h_ + (size_t)i * NO;
-    if (acc) { acc[i][0] = o[0]; acc[i][1] = o[1]; acc[i][2] = o[2]; }
-    if (pot) pot[i] = o[3];
-    if (jrk) { jrk[i][0] = o[4]; jrk[i][1] = o[5]; jrk[i][2] = o[6]; }
-    if (order_ >= ORDER_H6) {
-      if (snp) { snp[i][0] = o[7]; snp[i][1] = o[8]; snp[i][2] = o[9]; }
-      if (crk) { crk[i][0] = crk[i][1] = crk[i][2] = 0.0; }
-    }
-    if (ngb) {
+  // result rows are field-major in the host-mapped block
+  auto vec3 = [&](double (*dst)[3], int f0) {
+    if (!dst) return;
+    const double *x = result_field(f0), *y = result_field(f0 + 1), *z = result_field(f0 + 2);
+    for (int i = 0; i < ni; ++i) { dst[i][0] = x[i]; dst[i][1] = y[i]; dst[i][2] = z[i]; }
+  };
+  vec3(acc, 0);
+  if (pot) memcpy(pot, result_field(3), sizeof(double) * (size_t)ni);
+  vec3(jrk, 4);
+  if (order_ >= ORDER_H6) {
+    vec3(snp, 7);
+    if (crk)
+      for (int i = 0; i < ni; ++i) crk[i][0] = crk[i][1] = crk[i][2] = 0.0;
+  }
+  if (ngb) {
+    const double *ds = result_field(NF), *nn = result_field(NF + 1);
+    for (int i = 0; i < ni; ++i) {
       if (nnb) nnb[i] = 0;
-      const double ds = o[NF];
-      if (ds < 1.0e10) {
-        if (nnb) nnb[i] = (int)o[NF + 1];
-        if (dsmin) dsmin[i] = ds;
+      if (ds[i] < 1.0e10) {
+        if (nnb) nnb[i] = (int)nn[i];
+        if (dsmin) dsmin[i] = ds[i];
       }
     }
   }
   return 0;
 }
 
-// Reference read_ngb_list (sapporohostclass.cpp:804-835): fetch the lists of
-// the last call; returns 1 if any i-particle has >= 256 neighbours.
-// Queued behind a call's result copy (after its completion event, so the
-// result wait does not include it): packing of the neighbour lists and the
-// copy of their offsets.  read_ngb_list then waits for ev_off_ and copies only
-// the packed ids.
-int Engine::queue_list_pack(int ni) {
-  if (ni <= 0) return 0;
-  PackArgs pa{};
-  pa.ngb_cnt = cnt_d_;
-  pa.ngb_list = list_d_;
-  pa.ni = ni;
-  pa.offsets = off_d_;
-  pa.packed = packed_d_;
-  if (launch_pack(stream_, pa) != 0) return fail("pack launch", hipGetLastError());
-  HIPCHK(hipMemcpyAsync(off_h_, off_d_, sizeof(int) * (size_t)(ni + 1), hipMemcpyDeviceToHost, stream_),
-         "offset download");
-  HIPCHK(hipEventRecord(ev_off_, stream_), "event");
-  return 0;
-}
-
+// Reference read_ngb_list (sapporohostclass.cpp:804-835): the lists of the
+// last call; returns 1 if any i-particle has >= 256 neighbours.  The finalize
+// wrote them sorted and packed into host-mapped memory, so once the call is
+// complete nothing is copied, unless their total exceeded the packed buffer:
+// then the sorted device slots are copied back instead.
 int Engine::read_ngb_list() {
   if (!open_ || !have_results_) return 0;
   if (wait_results() != 0) return -1;
   const int ni = ni_last_;
-  const int NO = out_fields(order_);
-  const int NF = part_fields(order_);
-  if (ngb_on_last_) {
-    HIPCHK(hipEventSynchronize(ev_off_), "offset wait");
-    const int total = off_h_[ni];
-    if (total > 0)
-      HIPCHK(hipMemcpyAsync(packed_h_, packed_d_, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, stream_),
-             "list download");
-    HIPCHK(hipStreamSynchronize(stream_), "list sync");
-  } else {
-    for (int i = 0; i <= ni; ++i) off_h_[i] = 0;
+  const double* cnt = result_field(part_fields(order_) + 2);
+  int overflow = 0;
+  long total = 0;
+  for (int i = 0; i < ni; ++i) {
+    const int c = (int)cnt[i];
+    overflow |= c >= NGB_CAP;
+    total += std::min(c, NGB_CAP);
+  }
+  if (ngb_on_last_ && total > packed_cap_ && !lists_in_slots_) {
+    slot_lists_h_.resize((size_t)ni * NGB_CAP);
+    HIPCHK(hipMemcpyAsync(slot_lists_h_.data(), list_d_, sizeof(int) * (size_t)ni * NGB_CAP, hipMemcpyDeviceToHost,
+                          stream_), "list slot download");
+    HIPCHK(hipStreamSynchronize(stream_), "list slot sync");
+    lists_in_slots_ = true;
   }
   ngb_fetched_ = true;
-  int overflow = 0;
-  for (int i = 0; i < ni; ++i)
-    if (out_h_[(size_t)i * NO + NF + 2] >= NGB_CAP) overflow = 1;
   return overflow;
 }
 
@@ -1083,12 +1166,13 @@ int Engine::get_ngb_list(int ipipe, int maxlen, int* n, int* list) {
     if (r < 0) return r;
   }
   if (ipipe < 0 || ipipe >= ni_last_) return fail("get_ngb_list: ipipe out of range");
-  const int NO = out_fields(order_);
-  const int NF = part_fields(order_);
-  const int len = (int)out_h_[(size_t)ipipe * NO + NF + 2];
+  const int len = ngb_on_last_ ? (int)result_field(part_fields(order_) + 2)[ipipe] : 0;
   const int c = std::min(len, NGB_CAP);
   const int ncopy = std::max(0, std::min(c, maxlen));
-  if (list && ncopy > 0) memcpy(list, packed_h_ + off_h_[ipipe], sizeof(int) * (size_t)ncopy);
+  if (list && ncopy > 0) {
+    const int* src = lists_in_slots_ ? slot_lists_h_.data() + (size_t)ipipe * NGB_CAP : packed_h_ + loff_h_[ipipe];
+    memcpy(list, src, sizeof(int) * (size_t)ncopy);
+  }
   if (n) *n = len;
   const bool overflow = len >= maxlen;
   if (overflow)
@@ -1101,7 +1185,7 @@ int Engine::force_prediction(int nj) {
   if (!open_) return fail("force_prediction: library not open");
   HIPCHK(hipSetDevice(device_), "hipSetDevice");
   if (inflight_ && wait_results() != 0) return -1;
-  if (flush_updates(0) != 0) return -1;
+  if (flush_updates(0, predict_ ? std::max(nj, 0) : 0) != 0) return -1;
   if (predict_ && nj > 0) {
     if (predict_range(0, nj) != 0) return -1;
   }

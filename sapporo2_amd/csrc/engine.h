@@ -7,7 +7,9 @@
 //     in pinned host memory and sent with one asynchronous copy per flush,
 //     then scattered by a kernel (no per-particle map lookups on the device);
 //   * each force call is scatter -> [predict] -> force -> finalize on one HIP
-//     stream, followed by one asynchronous result copy; lasthalf waits.  The
+//     stream; the finalize writes the result rows and the sorted, packed
+//     neighbour lists straight into host-mapped pinned memory, so lasthalf and
+//     read_neighbour_list wait for one completion event and copy nothing.  The
 //     kernels take the call's values (block time, softening, counts) from a
 //     small device-side descriptor, so with graphs on the sequence is captured
 //     once per launch shape and replayed (one graph launch per call).
@@ -102,7 +104,7 @@ class Engine {
   int graph_count() const { return (int)graph_cache_.size(); }
   float last_force_ms();          // force kernel time of the last call (timing on)
   float last_predict_ms();        // predictor time of the last call (timing on, 0 if none)
-  float last_device_ms();         // scatter..finalize device time of the last call (timing on)
+  float last_device_ms();         // scatter..finalize (lists packed) device time of the last call (timing on)
   sapk::ForceConfig last_config() const { return last_cfg_; }
   int device() const { return device_; }
   int num_cus() const { return cus_; }
@@ -114,10 +116,10 @@ class Engine {
   int fail(const char* what, hipError_t e = hipSuccess);
   int ensure_store(int need);
   int ensure_partials(size_t entries);
-  int flush_updates(int ni_reset);
-  int queue_list_pack(int ni);
+  int flush_updates(int ni_reset, int pred_j1 = 0);
   int upload_updates(int* nupd);
-  int scatter_updates(int n, int ni_reset);
+  int copy_staged();
+  int scatter_updates(int n, int ni_reset, int pred_j1 = 0);
   int predict_range(int j0, int j1);
   int post_desc(int ni, int nupd, int ni_reset, double eps2);
   bool pack_i(int ni, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
@@ -129,6 +131,10 @@ class Engine {
                         int sblocks, bool rows);
   void drop_graphs();
   int wait_results();
+  int ensure_lists(int need);
+  int ensure_packed(long need);
+  sapk::FinalizeArgs finalize_args(int grid_ni, bool ngb_on) const;
+  const double* result_field(int f) const { return res_h_ + (size_t)f * res_stride_; }
   int fetch_j_host(int nj);
   int grow_staging(int need);
   int ensure_icap(int need);
@@ -150,9 +156,11 @@ class Engine {
     int cap = 0;
     hipEvent_t ev = nullptr;  // completion of the copy that last read it
     bool busy = false;
+    uint64_t seq = 0;         // position of ev among the stream's recorded events
   };
   Stage stage_[2];
   int cur_ = 0, nstaged_ = 0;
+  int flushed_ = 0;           // staged rows [0, flushed_) already copied to the device (early flush)
   std::vector<int> slot_of_;  // address -> staged slot or -1
   double* upd_d_ = nullptr;
   int upd_cap_d_ = 0;
@@ -166,10 +174,14 @@ class Engine {
   double* ibuf_d_ = nullptr; int* iid_d_ = nullptr;
   double* part_d_ = nullptr; double* pds2_d_ = nullptr; int* paddr_d_ = nullptr;
   size_t part_cap_ = 0;
-  double* out_d_ = nullptr; double* out_h_ = nullptr;
-  int* cnt_d_ = nullptr; int* list_d_ = nullptr;
-  int* off_d_ = nullptr; int* packed_d_ = nullptr;
-  int* off_h_ = nullptr; int* packed_h_ = nullptr;
+  // host-mapped pinned results and packed lists (host pointer, device alias)
+  double* res_h_ = nullptr; double* res_hd_ = nullptr; long res_stride_ = 0;
+  int* loff_h_ = nullptr; int* loff_hd_ = nullptr;
+  int* packed_h_ = nullptr; int* packed_hd_ = nullptr; long packed_cap_ = 0;
+  int* pack_ctr_d_ = nullptr;
+  int* cnt_d_ = nullptr; int* list_d_ = nullptr; int list_cap_ = 0;
+  std::vector<int> slot_lists_h_;   // device list slots read back when the packed lists did not fit
+  bool lists_in_slots_ = false;
   hipEvent_t ev_done_ = nullptr, ev_d0_ = nullptr, ev_d1_ = nullptr, ev_f0_ = nullptr, ev_f1_ = nullptr, ev_p0_ = nullptr, ev_p1_ = nullptr;
   bool inflight_ = false, have_results_ = false, ngb_fetched_ = false, ngb_on_last_ = false;
   bool timed_predict_ = false, timed_force_ = false;
@@ -187,8 +199,11 @@ class Engine {
   sapk::CallDesc* desc_h_ = nullptr;   // the descriptor slot at the start of the call block
   sapk::CallDesc* desc_d_ = nullptr;
   hipEvent_t ev_desc_ = nullptr;
-  hipEvent_t ev_off_ = nullptr;   // neighbour-list offsets copied (queued behind each call with lists)
   bool desc_pending_ = false;
+  uint64_t desc_seq_ = 0;
+  // recorded events are numbered in stream order; done_seq_ is the number of the
+  // latest completion event seen complete (everything recorded before it is done)
+  uint64_t rec_seq_ = 0, done_seq_ = 0, done_rec_seq_ = 0;
   bool graphs_ = false;
   uint64_t gen_ = 1;          // bumped whenever a buffer a captured graph uses is reallocated
   uint64_t tick_ = 0;
@@ -232,3 +247,9 @@ class Engine {
 };
 
 }  // namespace sap
+
+// The object behind a sapamd_engine handle (include/sapporo_amd.h); the g6
+// and 6th-order ABIs hold one as their global instance.
+struct sapamd_engine {
+  sap::Engine eng;
+};

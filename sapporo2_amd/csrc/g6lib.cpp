@@ -1,6 +1,7 @@
 // g6lib.cpp -- GRAPE-6 C-ABI (declared in include/g6lib.h) over one global
 // engine, as the reference's sapporoG6lib.cpp does with `sapporo grav` (line 3).
 #include "../../include/g6lib.h"
+#include "../../include/sapporo_amd.h"
 
 #include <cstdio>
 #include <vector>
@@ -9,11 +10,14 @@
 #include "engine.h"
 
 namespace {
-sap::Engine g_grav;
+sapamd_engine g_handle;
+sap::Engine& g_grav = g_handle.eng;
 std::vector<double> g_dsmin;  // reference dsmin_i scratch (sapporoG6lib.cpp:18, 41)
 }  // namespace
 
 extern "C" {
+
+sapamd_engine* sapamd_g6_engine(void) { return &g_handle; }
 
 int g6_open_special(int ndev, int* list) {
   const int prec = sap::precision_from_env(sapk::PREC_DS);

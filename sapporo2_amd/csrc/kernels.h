@@ -92,8 +92,11 @@ struct ScatterArgs {
   const double* upd;     // nupd rows of nf + 1 doubles: fields 0..nf-1 of RawField, then (address, id) as an int2
   int nf;
   int* ngb_cnt;          // zeroed for [0, ni_reset)
+  int* pack_ctr;         // neighbour-list pack counter, zeroed when ni_reset > 0 (may be null)
   const CallDesc* cd;
   int grid_blocks;       // launch size (host)
+  int pred_j1;           // a prediction of [0, pred_j1) follows in the same call: the predicted
+                         // records of updated addresses below it are left to the predictor
 };
 
 struct PredictArgs {
@@ -128,6 +131,11 @@ struct ForceConfig { int P; int S; int jchunk; int smajor; int pipe; };
 // splits of one i-group; they combine their sums in LDS and write one row.
 inline int partial_rows(const ForceConfig& c) { return (c.S % 4 == 0 && !c.smajor) ? c.S / 4 : c.S; }
 
+// Results and neighbour lists go straight to host-mapped pinned memory (no
+// copy command): field-major result rows out[f * ostride + i], and each sorted
+// list at packed[list_off[i] ...], its offset taken from pack_ctr (one atomic
+// per work-group).  A list that would end beyond packed_cap is not written to
+// the host; every sorted list is also kept in ngb_list (device) for that case.
 struct FinalizeArgs {
   const double* part; const double* part_ds2; const int* part_addr;
   int S;
@@ -136,7 +144,12 @@ struct FinalizeArgs {
   int rows;              // shape: 1 thread per i-particle (else a work-group per i-particle)
   const void* rec;       // to look up the neighbour's id
   int* ngb_cnt; int* ngb_list;
-  double* out;           // ni x out_fields
+  double* out;           // host-mapped: out_fields x ostride doubles, field-major
+  long ostride;
+  int* list_off;         // host-mapped: ni list offsets into packed
+  int* packed;           // host-mapped: sorted ids of all lists, packed_cap ints
+  long packed_cap;
+  int* pack_ctr;         // device: next free slot of packed (zeroed by the scatter)
   int ngb_on;
 };
 
@@ -156,12 +169,6 @@ struct EnsArgs {
   int* ngb_cnt; int* ngb_list; int ngb_on;
 };
 
-struct PackArgs {
-  const int* ngb_cnt; const int* ngb_list; int ni;
-  int* offsets;          // ni + 1
-  int* packed;           // sum(min(cnt, NGB_CAP))
-};
-
 // Gathers the FP64 predicted state (pos, mass, vel, [acc]) into a dense AoS
 // buffer for read-back, 10 doubles per row: of all j in [0, n) (addr null) or
 // of the n listed addresses.
@@ -171,7 +178,6 @@ int launch_scatter(hipStream_t s, int order, bool f64, const ScatterArgs& a);
 int launch_predict(hipStream_t s, int order, bool f64, const PredictArgs& a);
 int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const ForceConfig& c);
 int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a);
-int launch_pack(hipStream_t s, const PackArgs& a);
 // Ensemble force + finalize (+ list sort) over a work table whose first n64
 // items have lane shape P = 64; at most max_waves persistent waves per part.
 int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64,

@@ -15,8 +15,9 @@
 //                 positions, relative positions and pair terms in FP32,
 //                 per-tile FP32 sums folded into FP64 accumulators.
 //   k_finalize    deterministic fixed-order reduction of the S partial sums,
-//                 nearest-neighbour selection and neighbour-list sort.
-//   k_pack_*      compacts the per-i neighbour lists for read-back.
+//                 nearest-neighbour selection, neighbour-list sort and packing;
+//                 results and packed lists are written straight into
+//                 host-mapped pinned memory (no copy command per call).
 //
 // Built with -ffp-contract=off: every fused multiply-add below is explicit.
 #include "kernels.h"
@@ -72,6 +73,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
   const int stride = gridDim.x * blockDim.x;
   const int ni_reset = a.cd->ni_reset, nupd = a.cd->nupd;
   for (int k = tid; k < ni_reset; k += stride) a.ngb_cnt[k] = 0;
+  if (tid == 0 && ni_reset > 0 && a.pack_ctr) *a.pack_ctr = 0;
   const size_t cap = (size_t)a.st.cap;
   for (int k = tid; k < nupd; k += stride) {
     const double* u = a.upd + (size_t)k * (a.nf + 1);
@@ -84,7 +86,9 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) {
     // widen the tile's id range (conservative until the next prediction recomputes it)
     atomicMin(&a.st.idr[addr / JTILE].x, id);
     atomicMax(&a.st.idr[addr / JTILE].y, id);
-    // predicted = raw for updated particles (kernels.cu:281-298)
+    // predicted = raw for updated particles (kernels.cu:281-298), unless the
+    // predictor that follows rewrites this address anyway
+    if (addr < a.pred_j1) continue;
     double ax = 0, ay = 0, az = 0;
     if constexpr (ORDER >= ORDER_H6) { ax = u[RF_AX]; ay = u[RF_AY]; az = u[RF_AZ]; }
     put_rec<ORDER, F64>(a.st.rec, addr, u[RF_X], u[RF_Y], u[RF_Z], u[RF_M], u[RF_VX], u[RF_VY],
@@ -643,20 +647,22 @@ __global__ __launch_bounds__(256) void k_finalize_ens(FinalizeArgs f, EnsArgs a)
 //                    followed by k_sort_lists (one wave per list);
 //   k_finalize_block one work-group per i-particle, threads striding over the
 //                    splits (few i-particles, many splits).
+// Result row of i-particle i, field-major (consecutive i of a wave write
+// consecutive doubles of each field: whole lines over PCIe).
 template <int ORDER, bool F64>
 __device__ __forceinline__ void write_row(const FinalizeArgs& a, int i, const double* T, double ds, int ad,
                                           int cnt) {
   using Rec = typename RecOf<ORDER, F64>::T;
   constexpr int NF = nf<ORDER>();
-  constexpr int NO = out_fields(ORDER);
-  double* o = a.out + (size_t)i * NO;
+  double* o = a.out + i;
+  const size_t st = (size_t)a.ostride;
 #pragma unroll
-  for (int f = 0; f < NF; ++f) o[f] = T[f];
+  for (int f = 0; f < NF; ++f) o[f * st] = T[f];
   const bool has = ad != INT_MAX;
   // the device returns ds2 rounded to float (kernels.cu:999-1002)
-  o[NF] = has ? (double)(float)ds : INFINITY;
-  o[NF + 1] = has ? (double)reinterpret_cast<const Rec*>(a.rec)[ad].id : -1.0;
-  o[NF + 2] = (double)cnt;
+  o[NF * st] = has ? (double)(float)ds : INFINITY;
+  o[(NF + 1) * st] = has ? (double)reinterpret_cast<const Rec*>(a.rec)[ad].id : -1.0;
+  o[(NF + 2) * st] = (double)cnt;
 }
 
 // Rank sort of c ids staged in s[0..c) by one wave, written to dst.
@@ -714,17 +720,45 @@ __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
   finalize_rows_block<ORDER, F64>(a, blockIdx.x);
 }
 
-// One wave per i-particle: ascending sort of its (at most NGB_CAP) list.
+// Writes the sorted list s[0..c) of i-particle i: back into its device slots
+// and, when it fits, to the host-mapped packed buffer at off (consecutive lanes
+// write consecutive ids).
+__device__ __forceinline__ void store_list(const FinalizeArgs& a, int i, const int* s, int c, int off, int t,
+                                           int nt) {
+  const bool host = (long)off + c <= a.packed_cap;
+  for (int k = t; k < c; k += nt) {
+    const int v = s[k];
+    a.ngb_list[(size_t)i * NGB_CAP + k] = v;
+    if (host) a.packed[off + k] = v;
+  }
+}
+
+// One wave per i-particle: ascending sort of its (at most NGB_CAP) list in LDS,
+// then packing.  The four lists of a work-group take their packed offsets with
+// one atomic on the call's counter.
 __device__ __forceinline__ void sort_lists_block(const FinalizeArgs& a, int b) {
   __shared__ int lst[4][NGB_CAP];
+  __shared__ int srt[4][NGB_CAP];
+  __shared__ int cnts[4];
+  __shared__ int base;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int i = b * 4 + w;
-  const int c = i < a.cd->ni ? min(a.ngb_cnt[i], NGB_CAP) : 0;
-  if (c > 1)
-    for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
+  const bool valid = i < a.cd->ni;
+  const int c = valid ? min(a.ngb_cnt[i], NGB_CAP) : 0;
+  for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
+  if (lane == 0) cnts[w] = c;
   __syncthreads();
-  if (c > 1) wave_rank_sort(lst[w], c, lane, a.ngb_list + (size_t)i * NGB_CAP);
+  if (threadIdx.x == 0) {
+    const int tot = cnts[0] + cnts[1] + cnts[2] + cnts[3];
+    base = tot > 0 ? atomicAdd(a.pack_ctr, tot) : 0;
+  }
+  wave_rank_sort(lst[w], c, lane, srt[w]);
+  __syncthreads();
+  int off = base;
+  for (int u = 0; u < w; ++u) off += cnts[u];
+  if (valid && lane == 0) a.list_off[i] = off;
+  store_list(a, i, srt[w], c, off, lane, 64);
 }
 
 __global__ __launch_bounds__(256) void k_sort_lists(FinalizeArgs a) { sort_lists_block(a, blockIdx.x); }
@@ -747,6 +781,8 @@ __global__ __launch_bounds__(256) void k_finalize_block(FinalizeArgs a) {
   __shared__ double shD[256];
   __shared__ int shA[256];
   __shared__ int lst[NGB_CAP];
+  __shared__ int srt[NGB_CAP];
+  __shared__ int base;
   const int t = threadIdx.x;
   const int i = blockIdx.x;
   const int ni = a.cd->ni;
@@ -781,51 +817,23 @@ __global__ __launch_bounds__(256) void k_finalize_block(FinalizeArgs a) {
     __syncthreads();
   }
   const int cnt = a.ngb_on ? a.ngb_cnt[i] : 0;
+  const int c = min(cnt, NGB_CAP);
   if (t == 0) {
     double R[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) R[f] = shT[f][0];
     write_row<ORDER, F64>(a, i, R, shD[0], shA[0], cnt);
+    if (a.ngb_on) {
+      base = c > 0 ? atomicAdd(a.pack_ctr, c) : 0;
+      a.list_off[i] = base;
+    }
   }
-  const int c = min(cnt, NGB_CAP);
-  if (c <= 1) return;  // uniform over the block
+  if (c == 0) return;  // uniform over the block
   for (int k = t; k < c; k += 256) lst[k] = a.ngb_list[(size_t)i * NGB_CAP + k];
   __syncthreads();
-  if (t < 64) wave_rank_sort(lst, c, t, a.ngb_list + (size_t)i * NGB_CAP);
-}
-
-// ---------------------------------------------------------------------------
-// Neighbour-list packing: offsets by a single-block scan, then a copy.
-__global__ __launch_bounds__(1024) void k_pack_scan(PackArgs a) {
-  __shared__ int sm[1024];
-  const int t = threadIdx.x;
-  const int per = (a.ni + 1023) / 1024;
-  const int b = t * per, e = min(a.ni, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += min(a.ngb_cnt[i], NGB_CAP);
-  sm[t] = s;
+  if (t < 64) wave_rank_sort(lst, c, t, srt);
   __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int v = (t >= d) ? sm[t - d] : 0;
-    __syncthreads();
-    sm[t] += v;
-    __syncthreads();
-  }
-  int off = sm[t] - s;
-  for (int i = b; i < e; ++i) {
-    a.offsets[i] = off;
-    off += min(a.ngb_cnt[i], NGB_CAP);
-  }
-  if (t == 1023) a.offsets[a.ni] = sm[1023];
-}
-
-__global__ __launch_bounds__(256) void k_pack_copy(PackArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= a.ni) return;
-  const int c = min(a.ngb_cnt[i], NGB_CAP);
-  const int o = a.offsets[i];
-  for (int k = lane; k < c; k += 64) a.packed[o + k] = a.ngb_list[(size_t)i * NGB_CAP + k];
+  store_list(a, i, srt, c, base, t, 256);
 }
 
 // ---------------------------------------------------------------------------
@@ -979,13 +987,6 @@ int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const
     hipLaunchKernelGGL((k_finalize_ens<O, F>), dim3(blocks_for(f.grid_ni, 256)), dim3(256), 0, s, f, a);
     if (f.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(f.grid_ni, 4)), dim3(256), 0, s, f);
   });
-  return hip_status();
-}
-
-int launch_pack(hipStream_t s, const PackArgs& a) {
-  if (a.ni <= 0) return 0;
-  hipLaunchKernelGGL(k_pack_scan, dim3(1), dim3(1024), 0, s, a);
-  hipLaunchKernelGGL(k_pack_copy, dim3(blocks_for(a.ni, 4)), dim3(256), 0, s, a);
   return hip_status();
 }
 

@@ -23,8 +23,11 @@ HEADERS = {
 
 
 def declared_functions(header):
+    """Functions the header declares for the library to export (static inline
+    caller-side definitions excluded: they live in the header)."""
     txt = open(os.path.join(INC, header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = "\n".join(line for line in txt.splitlines() if not line.lstrip().startswith("static inline"))
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt))
     keywords = {"if", "defined", "sizeof", "return"}
     # keep only names that appear as declarations: "<type> name(" at line start
@@ -53,6 +56,66 @@ def test_header_symbols_exported(header):
         if n not in syms:
             missing.append(n)
     assert not missing, f"{os.path.basename(lib)} lacks {missing}"
+
+
+# The caller side of the reference header (testCodes/src/g6lib.h:52-90).
+G6_CALLER_SIDE = ["g6_close", "g6_open", "g6_npipes", "g6_set_ti", "g6_reset", "g6_reset_fofpga",
+                  "g6_set_j_particle", "g6calc_firsthalf", "g6calc_lasthalf", "g6_set_tunit", "g6_set_xunit",
+                  "g6_set_j_particle_multisend_mxfast_", "g6_reinitialize", "g6_print_chip_status",
+                  "g6_flush_jp_buffer_and_multisend", "g6_initialize_jp_buffer",
+                  "g6_set_overflow_flag_test_mode_"]
+
+
+def test_g6_header_has_reference_caller_side():
+    txt = open(os.path.join(INC, "g6lib.h")).read()
+    defined = set(re.findall(r"static inline [A-Za-z_ ]+?\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt))
+    missing = [n for n in G6_CALLER_SIDE if n not in defined]
+    assert not missing, f"g6lib.h lacks the caller-side definitions {missing}"
+
+
+def test_caller_side_wrappers_link_from_two_units(tmp_path):
+    """A C and a C++ translation unit of one program both include g6lib.h and call the
+    by-value wrappers; the program links against the library (the header's definitions
+    are static inline, so they do not collide)."""
+    (tmp_path / "a.c").write_text(
+        '#include "g6lib.h"\n'
+        "int unit_a(void) {\n"
+        "  double x[1][3] = {{0, 0, 0}}, v[1][3] = {{0, 0, 0}}, a[1][3], j[1][3], pot[1], h2[1] = {0};\n"
+        "  int id[1] = {0};\n"
+        "  g6_set_ti(0, 0.0);\n"
+        "  g6calc_firsthalf(0, 1, 1, id, x, v, a, j, pot, 1e-4, h2);\n"
+        "  return g6calc_lasthalf(0, 1, 1, id, x, v, 1e-4, h2, a, j, pot) + g6_npipes();\n"
+        "}\n")
+    (tmp_path / "b.cpp").write_text(
+        '#include "g6lib.h"\n'
+        'extern "C" int unit_a(void);\n'
+        "int main() {\n"
+        "  double m = 1, x[3] = {0, 0, 0}, z[3] = {0, 0, 0};\n"
+        "  int cl = 0, n = 1, addr = 0, id = 0;\n"
+        "  if (g6_open(0) != 0) return 1;\n"
+        "  g6_set_j_particle(0, 0, 0, 0.0, 0.0, 1.0, z, z, z, z, x);\n"
+        "  g6_set_j_particle_multisend_mxfast_(&cl, &n, &addr, &id, &m, x);\n"
+        "  int r = unit_a() + g6_reset(0) + g6_initialize_jp_buffer(0, 1);\n"
+        "  return g6_close(0) + (r < 0);\n"
+        "}\n")
+    objs = []
+    for src, cc, std in (("a.c", "gcc", "-std=c99"), ("b.cpp", "g++", "-std=c++11")):
+        obj = tmp_path / (src + ".o")
+        subprocess.check_call([cc, std, "-Wall", "-Werror", "-c", "-I", INC, str(tmp_path / src), "-o", str(obj)])
+        objs.append(str(obj))
+    exe = tmp_path / "two_units"
+    subprocess.check_call(["g++", "-o", str(exe)] + objs + ["-L", os.path.dirname(sa.LIB_PATH), "-lsapporo_amd"])
+    assert exe.exists()
+
+
+def test_wrapper_integrator_builds(tmp_path):
+    """The shared-step integrator written against the caller-side names only compiles
+    warning-free and links unchanged (tests/integrator/hermite4_wrap_g6.cpp)."""
+    exe = tmp_path / "h4w"
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-I", INC, "-o", str(exe),
+                           os.path.join(ROOT, "tests", "integrator", "hermite4_wrap_g6.cpp"),
+                           "-L", os.path.dirname(sa.LIB_PATH), "-lsapporo_amd"])
+    assert exe.exists()
 
 
 def test_engine_symbol_list_matches_header():

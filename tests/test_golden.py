@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
+from _util import check_forces
 
 GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
 
@@ -57,11 +58,9 @@ def test_gpu_matches_golden(path, prec):
         eng.start_grav_calc(n, g["ids"], g["x"], g["v"], float(g["eps2"]), g["h2"],
                             ai=g["a"] if order == 2 else None)
         r = eng.get_grav_results(n, n)
-        tol = 1e-5 if prec == 1 else 1e-12
-        for k in ("acc", "jrk", "pot") + (("snp",) if order == 2 else ()):
-            e = oracle.rel_err(r[k], g[k])
-            lim = tol * (30 if k in ("jrk", "snp") else 1)
-            assert np.median(e) <= lim, (k, np.median(e))
+        # a median over fewer than 5 particles is one particle's error: 10x
+        tol = (1e-6 if prec == 1 else 1e-12) * (10 if n < 5 else 1)
+        check_forces(r, g, order, tol, label=os.path.basename(path))
         pre = "mx_" if prec == 1 else ""
         nnb, dsmin, found = oracle.api_nn(g[pre + "nn_addr"], g[pre + "nn_ds2"], g["ids"])
         assert np.array_equal(r["nnb"], nnb)

@@ -246,6 +246,36 @@ def test_integrator_driver_energy():
     assert finals["double"] < 1e-5 and finals["mixed"] < 1e-5, finals
 
 
+def test_wrapper_integrator_caller_side():
+    """The integrator written against the header's caller side only (by-value wrappers,
+    position-only multisend, board stubs; the call pattern of test_integrator.cpp:94-96,
+    218, 238, 267): stubs return 0, a position-only load gives acc and pot bit-identical
+    to the full load (jerk differs: vj = 0), and energy is conserved."""
+    _say("caller-side wrapper integrator")
+    exe = os.path.join(ROOT, "tests", "integrator", "hermite4_wrap_g6")
+    assert os.path.exists(exe), "build tests/integrator first (__graft_entry__.build)"
+    n = 1024
+    m, x, v = ics.plummer(n, seed=53)
+    path = os.path.join(os.environ.get("TMPDIR", "/tmp"), "h4w_input_%d.bin" % os.getpid())
+    with open(path, "wb") as f:
+        np.array([n], np.int32).tofile(f)
+        np.concatenate([m[:, None], x, v], axis=1).astype(np.float64).tofile(f)
+    for prec in ("mixed", "double"):
+        env = dict(os.environ, SAPPORO2_AMD_PRECISION=prec)
+        out = subprocess.run([exe, path, "0.125", "0.02", str(4.0 / n)], capture_output=True, text=True,
+                             timeout=120, env=env)
+        assert out.returncode == 0, out.stderr[-2000:]
+        lines = out.stdout.splitlines()
+        stubs = [l for l in lines if l.startswith("stubs")][0].split()
+        fast = [l for l in lines if l.startswith("mxfast")][0].split()
+        last = [l for l in lines if l.startswith("final")][-1]
+        _say(f"wrapper integrator {prec}: {' '.join(stubs)} | {' '.join(fast)} | {last}")
+        assert stubs[1] == "0" and stubs[3] == "16384", stubs
+        assert fast[2] == "1" and fast[4] == "1", fast
+        assert abs(float(last.split()[-1])) < 1e-5, last
+    os.remove(path)
+
+
 def test_block_timestep_integrator_energy():
     """The external block-timestep Hermite-4 integrator (individual power-of-two steps,
     active sets from single particles to all) conserves energy through the g6 ABI."""

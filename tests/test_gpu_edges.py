@@ -22,7 +22,7 @@ from _util import check_forces, check_nn_lists, load_engine, plummer_system
 
 pytestmark = pytest.mark.gpu
 PRECS = [sa.DOUBLESINGLE, sa.DOUBLE]
-TOL = {sa.DOUBLESINGLE: 1e-5, sa.DOUBLE: 1e-12}
+TOL = {sa.DOUBLESINGLE: 1e-6, sa.DOUBLE: 1e-12}
 
 
 def _say(msg):

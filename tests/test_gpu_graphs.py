@@ -95,7 +95,7 @@ def test_graph_replays_match_direct(order, prec):
                 ref = oracle.force(order, x[sel], v[sel], ids[sel], xj, vj, mj, idj, eps2, h2=h2[sel],
                                    ai=acc[sel] if order >= sa.SIXTH else None,
                                    aj=pa if order >= sa.SIXTH else None)
-                tol = 1e-5 if prec == sa.DOUBLESINGLE else 1e-12
+                tol = 1e-6 if prec == sa.DOUBLESINGLE else 1e-12
                 check_forces(res[1], ref, order, tol * (10 if order >= sa.SIXTH else 1), label=f"graph call {c}")
                 if order < sa.SIXTH:
                     check_nn_lists(engs[1], res[1], prec == sa.DOUBLE, x[sel], ids[sel], xj, idj, h2[sel],

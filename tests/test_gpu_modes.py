@@ -14,7 +14,7 @@ import sapporo2_amd as sa
 from _util import check_forces, check_nn_lists, load_engine, plummer_system
 
 pytestmark = pytest.mark.gpu
-TOL = {sa.DOUBLESINGLE: 1e-5, sa.DOUBLE: 1e-12}
+TOL = {sa.DOUBLESINGLE: 1e-6, sa.DOUBLE: 1e-12}
 
 
 @pytest.mark.parametrize("prec", [sa.DOUBLESINGLE, sa.DOUBLE])

@@ -459,6 +459,8 @@ def main():
                     help="only run the known-size copy used to calibrate traffic counters")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="run one kernel only (profiling)")
+    ap.add_argument("--write-workload", metavar="PATH", default=None,
+                    help="only write the block-step workload file for the g6 driver (to profile it directly)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine drives one GPU; --gpus {args.gpus} is not supported "
@@ -466,6 +468,13 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.write_workload:
+        w = Workload(args.n, SEED)
+        write_workload(args.write_workload, w.n, w.ids, w.m, w.x, w.v, w.acc, w.jrk, w.tj, w.h2, w.px, w.pv,
+                       w.blocks, EPS2, w.resend, w.ngb)
+        print(json.dumps({"workload": args.write_workload, "blocks": len(w.blocks),
+                          "sample_block": len(w.blocks) - 1}), flush=True)
         return
     steps, warmup = max(1, args.steps), max(0, args.warmup)
     nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -141,6 +141,30 @@ int sapamd_synchronize(sapamd_engine *e);
  * above.  No reference counterpart (the reference's instance is file-static,
  * sapporoG6lib.cpp:3).  Valid for the life of the process; do not destroy. */
 sapamd_engine *sapamd_g6_engine(void);
+
+/* Allocation-free variant of sapamd_start_grav_calc + sapamd_get_grav_results,
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  No counterpart in the reference.  It calls no hipMalloc,
+ * hipFree or allocating routine and never waits on the host: the work is
+ * queued on `stream` (a hipStream_t; NULL = the engine's own stream) and
+ * complete when that stream is.
+ *   id   : device pointer, ni ids
+ *   ibuf : device pointer, [xi | vi | ai (SIXTH only)], each ni x 3 doubles
+ *   out  : device pointer, field-major ni-strided rows: acc x, y, z, pot,
+ *          jerk x, y, z, [snap x, y, z,] nearest-neighbour ds2 (float-rounded;
+ *          +inf when none), nearest-neighbour id (-1 when none), 0
+ * No neighbour lists.  0 < ni <= 16384 and nj within the j-store; staged
+ * j-updates must have been flushed (sapamd_force_prediction) before, and a
+ * pending prediction (sapamd_set_time) runs inside the call.  The call
+ * descriptor and partial sums use a per-GPU device array of
+ * sapamd_noalloc_scratch_bytes() bytes that the library holds until the
+ * process ends; a call whose partial sums would not fit coarsens its j-split
+ * (results then differ from sapamd_start_grav_calc by rounding only), and
+ * one that still does not fit fails with -1.  All calls on one GPU share the
+ * array: calls that overlap on one GPU race.  Returns 0 or -1. */
+int sapamd_grav_calc_noalloc(sapamd_engine *e, int nj, int ni, const int *id, const double *ibuf,
+                             double eps2, double *out, void *stream);
+long long sapamd_noalloc_scratch_bytes(void);
 const char *sapamd_last_error(sapamd_engine *e);
 
 #ifdef __cplusplus

@@ -55,7 +55,8 @@ ENGINE_SYMBOLS = [
     "sapamd_state_bytes", "sapamd_save_state", "sapamd_load_state",
     "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
     "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
-    "sapamd_start_grav_calc_clusters", "sapamd_g6_engine",
+    "sapamd_start_grav_calc_clusters", "sapamd_g6_engine", "sapamd_grav_calc_noalloc",
+    "sapamd_noalloc_scratch_bytes",
 ]
 
 

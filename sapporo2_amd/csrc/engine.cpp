@@ -988,6 +988,84 @@ int Engine::enqueue_call(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_o
   return 0;
 }
 
+// The allocation-free call: same launch shape as start_grav_calc (the j-split
+// is coarsened only when its partial rows would not fit the scratch region),
+// same kernels, everything queued on s.  Preconditions instead of the
+// allocating steps of a regular call: no staged j-updates and nj within the
+// store (flush with force_prediction first).
+int Engine::grav_calc_scratch(void* scratch, size_t bytes, int nj, int ni, const int* id_d, const double* ibuf_d,
+                              double eps2, double* out_d, hipStream_t s) {
+  if (!open_) return fail("grav_calc_noalloc: library not open");
+  if (ni <= 0 || nj <= 0 || ni > NPIPES) return fail("grav_calc_noalloc: need 0 < ni <= 16384 and nj > 0");
+  if (!scratch || !id_d || !ibuf_d || !out_d) return fail("grav_calc_noalloc: null pointer");
+  if (nj > st_.cap) return fail("grav_calc_noalloc: nj beyond the j-store (set and flush the j-particles first)");
+  if (nstaged_ > 0) return fail("grav_calc_noalloc: staged j-updates pending (flush with force_prediction first)");
+  HIPCHK(hipSetDevice(device_), "hipSetDevice");
+  if (!s) s = stream_;
+  const int NF = part_fields(order_);
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  ForceConfig cfg = choose_config(ni, nj, false);
+  auto need = [&](const ForceConfig& c) {
+    const size_t r = (size_t)partial_rows(c) * ni;
+    return al(sizeof(CallDesc)) + al(sizeof(double) * NF * r) + al(sizeof(double) * r) + al(sizeof(int) * r);
+  };
+  while (need(cfg) > bytes && cfg.S > 1) {
+    cfg.jchunk *= 2;
+    cfg.S = (nj + cfg.jchunk - 1) / cfg.jchunk;
+  }
+  if (need(cfg) > bytes) return fail("grav_calc_noalloc: call too large for the scratch region");
+  const size_t r = (size_t)partial_rows(cfg) * ni;
+  char* p = static_cast<char*>(scratch);
+  CallDesc* cd = reinterpret_cast<CallDesc*>(p);
+  p += al(sizeof(CallDesc));
+  double* part = reinterpret_cast<double*>(p);
+  p += al(sizeof(double) * NF * r);
+  double* pds2 = reinterpret_cast<double*>(p);
+  p += al(sizeof(double) * r);
+  int* paddr = reinterpret_cast<int*>(p);
+  const CallDesc v{t_i_, eps2, ni, 0, 0, 0};
+  if (launch_write_desc(s, cd, v) != 0) return fail("descriptor launch", hipGetLastError());
+  if (predict_) {
+    PredictArgs pa{};
+    pa.st = st_;
+    pa.j0 = 0;
+    pa.j1 = nj;
+    pa.cd = cd;
+    if (launch_predict(s, order_, f64_, pa) != 0) return fail("predict launch", hipGetLastError());
+    predict_ = false;
+    jhost_valid_ = false;
+  }
+  ForceArgs fa{};
+  fa.rec = st_.rec;
+  fa.idr = st_.idr;
+  fa.nj = nj;
+  fa.ibuf = ibuf_d;
+  fa.iid = id_d;
+  fa.cd = cd;
+  fa.grid_ni = ni;
+  fa.part = part;
+  fa.part_ds2 = pds2;
+  fa.part_addr = paddr;
+  fa.ngb_on = 0;
+  if (launch_force(s, order_, f64_, fa, cfg) != 0) return fail("force launch", hipGetLastError());
+  int bucket = 1;
+  while (bucket < ni) bucket <<= 1;
+  FinalizeArgs fz{};
+  fz.part = part;
+  fz.part_ds2 = pds2;
+  fz.part_addr = paddr;
+  fz.S = partial_rows(cfg);
+  fz.cd = cd;
+  fz.grid_ni = ni;
+  fz.rows = bucket >= 1024 ? 1 : 0;
+  fz.rec = st_.rec;
+  fz.out = out_d;
+  fz.ostride = ni;
+  fz.ngb_on = 0;
+  if (launch_finalize(s, order_, f64_, fz) != 0) return fail("finalize launch", hipGetLastError());
+  return 0;
+}
+
 constexpr size_t MAX_GRAPHS = 64;
 
 // Replays the captured sequence of this launch shape, capturing it first if

@@ -65,6 +65,13 @@ class Engine {
   int start_grav_calc_clusters(int ni, const int* cluster, const int* id, const double (*xi)[3],
                                const double (*vi)[3], const double (*ai)[3], const double* eps2_per_cluster,
                                const double* h2);
+  // A force call on device-resident inputs and outputs that allocates nothing
+  // and never waits on the host: the call descriptor and the partial sums live
+  // in the device region [scratch, scratch + bytes) (sapamd_grav_calc_noalloc).
+  // ibuf_d: [xi | vi | (ai, SIXTH)] (ni x 3 each); out_d: out_fields x ni
+  // doubles, field-major; no neighbour lists.  Work is queued on s.
+  int grav_calc_scratch(void* scratch, size_t bytes, int nj, int ni, const int* id_d, const double* ibuf_d,
+                        double eps2, double* out_d, hipStream_t s);
   int read_ngb_list();
   int get_ngb_list(int ipipe, int maxlen, int* n, int* list);
   int force_prediction(int nj);

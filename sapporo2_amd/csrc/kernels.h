@@ -183,6 +183,9 @@ int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a);
 int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64,
                      long max_waves);
 int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a);
+// Writes the call descriptor v to device memory cd with a one-thread kernel
+// (no copy command, nothing allocated).
+int launch_write_desc(hipStream_t s, CallDesc* cd, const CallDesc& v);
 
 // The P values with compiled force kernels.
 bool force_p_supported(int P);

@@ -863,6 +863,8 @@ __global__ __launch_bounds__(256) void k_gather_pred(GatherArgs a) {
   }
 }
 
+__global__ void k_write_desc(CallDesc* cd, CallDesc v) { *cd = v; }
+
 // ---------------------------------------------------------------------------
 // Launchers.
 static inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -996,6 +998,11 @@ int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a) 
   dispatch_op(order, f64, [&](auto OR, auto FP) {
     hipLaunchKernelGGL((k_gather_pred<decltype(OR)::value, decltype(FP)::value>), dim3(nb), dim3(256), 0, s, a);
   });
+  return hip_status();
+}
+
+int launch_write_desc(hipStream_t s, CallDesc* cd, const CallDesc& v) {
+  hipLaunchKernelGGL(k_write_desc, dim3(1), dim3(1), 0, s, cd, v);
   return hip_status();
 }
 

@@ -188,6 +188,8 @@ int main(int argc, char** argv) {
 
   // initial j-particles: raw state with their own times
   for (int i = 0; i < w.n; ++i) send_j(w, i, w.tj[i], w.x.data(), w.v.data());
+  fprintf(stderr, "g6_blocksteps: %d j-particles sent; %d + %d steps of %d blocks\n", w.n, warmup, steps,
+          w.nblocks);
 
   // kept sample: results of sample_block in the last timed step
   const size_t ns = w.blocks[sample_block].size();
@@ -273,6 +275,7 @@ int main(int argc, char** argv) {
     }
   }
   const double run_s = secs(run0, Clock::now());
+  fprintf(stderr, "g6_blocksteps: timed loop done (%.3f s)\n", run_s);
 
   // predicted j-state the force calls used (raw and predicted read-back)
   std::vector<double> ppos(3 * (size_t)w.n), pvel(3 * (size_t)w.n);
@@ -282,6 +285,7 @@ int main(int argc, char** argv) {
     get_j_part_data(i, w.n, pos, vel, a, jk, &ppos[3 * (size_t)i], &pvel[3 * (size_t)i], mass, e2, idv);
   }
   g6_close_(&cl);
+  fprintf(stderr, "g6_blocksteps: read-back done, library closed\n");
 
   // results file
   {
@@ -362,5 +366,7 @@ int main(int argc, char** argv) {
             1e3 * median(fst), 1e3 * median(wt), 1e3 * median(rd));
   }
   fprintf(o, "]}\n");
+  fflush(o);
+  fprintf(stderr, "g6_blocksteps: done\n");
   return 0;
 }

@@ -147,6 +147,7 @@ int Engine::open(int device, int order, int precision) {
   nj_max_ = NJ_INITIAL;
   max_addr_ = -1;
   slot_of_.assign(nj_max_, -1);
+  staged_addr_.clear();
   nstaged_ = 0;
   cur_ = 0;
   predict_ = false;
@@ -198,6 +199,7 @@ int Engine::close() {
   stream_ = nullptr;
   open_ = inflight_ = have_results_ = ngb_fetched_ = jhost_valid_ = false;
   slot_of_.clear();
+  staged_addr_.clear();
   nstaged_ = 0;
   return 0;
 }
@@ -328,6 +330,7 @@ int Engine::set_j_particle(int addr, int id, double tj, double dtj, double mass,
     }
     k = nstaged_++;
     slot_of_[addr] = k;
+    staged_addr_.push_back(addr);
   } else if (k < flushed_) {
     flushed_ = k;  // already copied: copy again from this row
   }
@@ -404,17 +407,12 @@ int Engine::set_j_particles(int n, const int* addr, const int* id, const double*
 // scatter kernel is launched separately so callers can bracket device work.
 int Engine::upload_updates(int* nupd) {
   const int n = nstaged_;
-  const int nr = raw_fields(order_) + 1;
   *nupd = n;
   if (n <= 0) return 0;
   if (ensure_store(max_addr_ + 1) != 0) return -1;
   if (copy_staged() != 0) return -1;
-  Stage& sg = stage_[cur_];
-  for (int k = 0; k < n; ++k) {
-    int pair[2];
-    memcpy(pair, sg.d + (size_t)k * nr + nr - 1, sizeof(pair));
-    slot_of_[pair[0]] = -1;
-  }
+  for (const int a : staged_addr_) slot_of_[a] = -1;
+  staged_addr_.clear();
   nstaged_ = 0;
   flushed_ = 0;
   cur_ ^= 1;

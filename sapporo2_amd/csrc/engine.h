@@ -169,6 +169,7 @@ class Engine {
   int cur_ = 0, nstaged_ = 0;
   int flushed_ = 0;           // staged rows [0, flushed_) already copied to the device (early flush)
   std::vector<int> slot_of_;  // address -> staged slot or -1
+  std::vector<int> staged_addr_;  // addresses of the staged rows, in row order
   double* upd_d_ = nullptr;
   int upd_cap_d_ = 0;
 

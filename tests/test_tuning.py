@@ -29,11 +29,16 @@ def _rows():
 
 
 def test_table_covers_every_bucket():
+    """Every tuned system size log2(nj) has one entry per (order, precision, lists,
+    log2 ni) bucket, for ni up to 16384 and up to nj."""
     rows = _rows()
-    keys = {(o, p, g, li) for o, p, g, li, _, _, _ in rows}
-    want = {(o, p, g, li) for o in (1, 2) for p in (1, 2) for g in (0, 1) for li in range(15)}
-    assert keys == want
-    assert len(rows) == len(keys)                  # one entry per bucket
+    ljs = sorted({r[4] for r in rows})
+    assert 17 in ljs                               # the block-step metric's N = 131072
+    for lj in ljs:
+        keys = [(o, p, g, li) for o, p, g, li, l, _, _ in rows if l == lj]
+        want = {(o, p, g, li) for o in (1, 2) for p in (1, 2) for g in (0, 1) for li in range(min(15, lj + 1))}
+        assert set(keys) == want, lj
+        assert len(keys) == len(want), lj          # one entry per bucket
 
 
 def test_table_shapes_are_valid():
@@ -51,11 +56,22 @@ def test_table_shapes_are_valid():
 def test_candidates_within_bounds(ni, nj):
     cands = autotune.candidates(ni, nj)
     assert cands, (ni, nj)
-    # the 128-wave floor holds for every candidate unless none can meet it (then it is dropped)
-    floor = 128 if any(math.ceil(ni / P) * S >= 128 for P, S in cands) else 1
+    # the wave floor holds for every candidate unless none can meet it (then it is dropped)
+    fl = autotune.MIN_WAVES
+    floor = fl if any(math.ceil(ni / P) * S >= fl for P, S in cands) else 1
     for P, S in cands:
         assert P in P_COMPILED
         Q = 64 // P
         assert math.ceil(nj / S) >= max(64, 16 * Q)
         waves = math.ceil(ni / P) * S
         assert floor <= waves <= 262144
+
+
+def test_select_prefers_fewer_partial_rows_within_tie():
+    """Among shapes within 1 % of the fastest the tuner keeps the one writing the fewest
+    partial-sum rows; a shape more than 1 % faster wins regardless."""
+    flat = [(1.5839, 64, 512, 256), (1.5849, 64, 256, 512), (1.5878, 64, 128, 1024), (1.6423, 64, 64, 2048)]
+    assert autotune.select(flat)[2] == 128
+    steep = [(3.244, 64, 1024, 256), (3.295, 64, 512, 512), (3.3935, 64, 256, 1024)]
+    assert autotune.select(steep)[2] == 1024
+    assert autotune.partial_rows(256) == 64 and autotune.partial_rows(6) == 6

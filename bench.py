@@ -48,6 +48,13 @@ METRIC = "Ginteractions/s Hermite-4 N=128K, mixed vs FP64 GPU at rel. force err 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "force_h4_mixed_traffic.md")
 DRIVER = os.path.join(ROOT, "sapporo2_amd", "bench", "g6_blocksteps")
 FLOP_PER_INTERACTION = 60.0          # test_gravity_block.cpp:271
+# Algorithmic bytes of the HBM-bound kernels (Hermite-4).  Predictor, per j-particle: reads
+# t, m, x, v, a, j (14 doubles) and the id; writes the working-precision record (48 B mixed,
+# 64 B FP64) and, mixed only, the FP64 predicted position and velocity kept for read-back.
+# Scatter, per staged update followed by a prediction in the same call: reads the staged row
+# (16 doubles and the address/id pair), writes the 16 raw fields and the id.
+PREDICT_BYTES = {"mixed": 116 + 48 + 48, "fp64": 116 + 64}
+SCATTER_BYTES = 136 + 128 + 4
 PEAK_TFLOPS = {"mixed": 157.3, "fp64": 78.6}   # MI355X FP32 / FP64 vector
 PRECS = (("mixed", sa.DOUBLESINGLE), ("fp64", sa.DOUBLE))
 EPS2 = 1e-6
@@ -188,7 +195,18 @@ def summarize(js, prec):
     dev_ms = float(np.median(js["step_device_ms"]))
     f = js["force64"]
     achieved = FLOP_PER_INTERACTION * f["interactions"] / (f["ms"] * 1e-3) / 1e12 if f["ms"] > 0 else 0.0
-    return dict(value=ips / api / 1e9, ms_per_step=api * 1e3,
+    pr, sc = js.get("predict", {}), js.get("scatter", {})
+    pred = scat = None
+    if pr.get("launches"):
+        t = pr["ms"] * 1e-3 / pr["launches"]
+        pred = dict(launches=pr["launches"], mean_us=round(t * 1e6, 2), bytes_per_j=PREDICT_BYTES[prec],
+                    nj=pr["nj"], achieved_GBps=round(PREDICT_BYTES[prec] * pr["nj"] / t / 1e9, 1))
+    if sc.get("launches"):
+        t = sc["ms"] * 1e-3 / sc["launches"]
+        u = sc["updates"] / sc["launches"]
+        scat = dict(launches=sc["launches"], mean_us=round(t * 1e6, 2), updates_per_launch=round(u, 1),
+                    bytes_per_update=SCATTER_BYTES, achieved_GBps=round(SCATTER_BYTES * u / t / 1e9, 1))
+    return dict(value=ips / api / 1e9, ms_per_step=api * 1e3, predictor=pred, scatter=scat,
                 device_value=ips / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else None, device_ms_per_step=dev_ms,
                 force_mean_ms=f["ms"] / max(1, f["launches"]), force_launches=f["launches"],
                 achieved_tflops=achieved, peak=PEAK_TFLOPS[prec],
@@ -197,6 +215,47 @@ def summarize(js, prec):
                 outside_api_ms_per_step={"j_resend": round(1e3 * float(np.median(js["step_resend_s"])), 3),
                                          "list_copy_out": round(1e3 * float(np.median(js["step_list_copy_s"])), 3)},
                 blocks=js["blocks"])
+
+
+def run_engine_loop(w, name, prec, steps, warmup):
+    """The same block-step sweep through the engine handle API inside this process (no
+    child process; the form profiled with rocprofv3): per-call device windows and force
+    kernel times from the engine's HIP events; prints one JSON line."""
+    eng = sa.Engine(sa.FOURTH, prec)
+    eng.set_timing(True)
+    n = w.n
+    eng.set_j_particles(np.arange(n), w.ids, w.m, w.x, w.v, w.acc, w.jrk, tj=w.tj)
+    api, dev, f_ms, f_int = [], [], 0.0, 0.0
+    for step in range(warmup + steps):
+        timed = step >= warmup
+        a_s, d_ms = 0.0, 0.0
+        for b, sel in enumerate(w.blocks):
+            prev = w.blocks[b - 1]
+            eng.set_j_particles(prev, w.ids[prev], w.m[prev], w.px[prev], w.pv[prev], w.acc[prev], w.jrk[prev],
+                                tj=0.0)
+            eng.set_time(0.0)
+            for c0 in range(0, len(sel), sa.NPIPES):
+                ch = sel[c0:c0 + sa.NPIPES]
+                t0 = time.perf_counter()
+                eng.start_grav_calc(n, w.ids[ch], w.px[ch], w.pv[ch], EPS2, w.h2[ch])
+                eng.get_grav_results(n, len(ch), ngb=True)
+                eng.read_ngb_list()
+                a_s += time.perf_counter() - t0
+                d_ms += eng.last_device_ms()
+                if timed and eng.last_config()[0] == 64:
+                    f_ms += eng.last_force_ms()
+                    f_int += len(ch) * n
+        if timed:
+            api.append(a_s)
+            dev.append(d_ms)
+    eng.close()
+    out = {"mode": "engine-loop", "kernel": name, "steps": steps, "warmup": warmup,
+           "ms_per_step_api": round(1e3 * float(np.median(api)), 4),
+           "ms_per_step_device": round(float(np.median(dev)), 4),
+           "device_value": round(w.interactions_per_step / (float(np.median(dev)) * 1e-3) / 1e9, 3),
+           "force64_ginter": round(f_int / (f_ms * 1e-3) / 1e9, 3) if f_ms > 0 else None}
+    print(json.dumps(out), flush=True)
+    return out
 
 
 def err_stats(got, ref, keys):
@@ -342,8 +401,10 @@ def extra_h6_king(nthreads, cpu_budget):
 def extra_ensemble(nthreads, cpu_budget):
     """512 Plummer clusters, N log-uniform in [256, 16384] (seed 4), eps2 = 1e-4: eight calls
     (active fractions 2^0..2^-7 of every cluster), one ensemble launch each, through the
-    engine handle API; errors and NN/list checks on the first two i of every cluster in the
-    all-active call against the FP64 oracle."""
+    engine handle API, forces and nearest neighbours (no neighbour lists, as the
+    configuration has none); errors and NN checks on the first two i of every cluster in the
+    all-active call against the FP64 oracle, and the lists of those i-particles from one
+    further, untimed all-active call with neighbour radii."""
     import oracle
     sizes = ics.ensemble_sizes(512, 256, 16384, seed=4)
     cl = [ics.plummer(int(k), seed=1000 + c) for c, k in enumerate(sizes)]
@@ -370,22 +431,24 @@ def extra_ensemble(nthreads, cpu_budget):
                 eng.set_j_particles(offs[c] + np.arange(len(mc)), np.arange(len(mc)), mc, xc, vc)
             eng.set_time(0.0)
             walls, devs = [], []
+            first = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            smp = np.concatenate([first + q for q in (0, 1)])
             for rep in range(3):
                 t0 = time.perf_counter()
                 dev = 0.0
                 for k, (cidx, I, X, V, H, _) in enumerate(calls):
-                    eng.start_grav_calc_clusters(cidx, I, X, V, eps, H)
+                    eng.start_grav_calc_clusters(cidx, I, X, V, eps)
                     r = eng.get_grav_results(1, len(I), ngb=True)
-                    eng.read_ngb_list()
                     dev += eng.last_device_ms()
                     if rep == 0 and k == 0:
-                        first = np.concatenate([[0], np.cumsum(sizes)[:-1]])
-                        smp = np.concatenate([first + q for q in (0, 1)])
-                        lists = [eng.get_ngb_list(int(i)) for i in smp]
-                        kept[name] = dict(smp=smp, res={q: r[q][smp] for q in ("acc", "jrk", "pot", "nnb")},
-                                          lists=lists)
+                        kept[name] = dict(smp=smp, res={q: r[q][smp] for q in ("acc", "jrk", "pot", "nnb")})
                 walls.append(time.perf_counter() - t0)
                 devs.append(dev)
+            cidx, I, X, V, H, _ = calls[0]
+            eng.start_grav_calc_clusters(cidx, I, X, V, eps, H)
+            eng.get_grav_results(1, len(I), ngb=True)
+            eng.read_ngb_list()
+            kept[name]["lists"] = [eng.get_ngb_list(int(i)) for i in smp]
             out[name] = dict(value=inter / np.median(walls[1:]) / 1e9, device=inter / (np.median(devs[1:]) * 1e-3) / 1e9)
     # oracle per cluster on its two sampled i-particles
     errs_in = {name: {q: [] for q in ("acc", "jrk", "pot")} for name in out}
@@ -459,6 +522,8 @@ def main():
                     help="only run the known-size copy used to calibrate traffic counters")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="run one kernel only (profiling)")
+    ap.add_argument("--engine-loop", action="store_true",
+                    help="run the block-step sweep through the engine handle API in this process only")
     ap.add_argument("--write-workload", metavar="PATH", default=None,
                     help="only write the block-step workload file for the g6 driver (to profile it directly)")
     args = ap.parse_args()
@@ -477,6 +542,12 @@ def main():
                           "sample_block": len(w.blocks) - 1}), flush=True)
         return
     steps, warmup = max(1, args.steps), max(0, args.warmup)
+    if args.engine_loop:
+        w = Workload(args.n, SEED)
+        for name, prec in PRECS:
+            if not args.only or args.only == name:
+                run_engine_loop(w, name, prec, steps, warmup)
+        return
     nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     tmp = tempfile.mkdtemp(prefix="sapamd_bench_")
     oracle_build = None
@@ -536,8 +607,12 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(r["achieved_tflops"], 3),
                      "peak": r["peak"], "unit": "TFLOP/s",
                      "frac": round(r["achieved_tflops"] / r["peak"], 4),
-                     "traffic": measured_traffic() if main_name == "mixed" else None,
-                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+                     "traffic": None,
+                     "traffic_note": "not measured for the current launch shapes (rocprofv3 counter collection "
+                                     "ends in a host SIGSEGV in a profiler thread with this build); last "
+                                     "measurement, at S = 256 j-splits per 16384-particle call: "
+                                     f"{measured_traffic()} B per launch, "
+                                     f"{os.path.relpath(TRAFFIC_PROFILE, ROOT)}",
                      "kernel": "k_force<H4,%s,P=64>" % main_name,
                      "mean_launch_ms": round(r["force_mean_ms"], 4),
                      "launches": r["force_launches"],
@@ -548,6 +623,8 @@ def main():
         "host_ms_per_step": r["host_ms_per_step"],
         "outside_api_ms_per_step": r["outside_api_ms_per_step"],
         "blocks": r["blocks"],
+        "predictor": r["predictor"],
+        "scatter": r["scatter"],
     }
     if "fp64" in res and main_name == "mixed":
         d = res["fp64"]

@@ -125,6 +125,9 @@ int sapamd_retrieve_predicted_j_particles(sapamd_engine *e, int n, const int *ad
 int sapamd_graph_count(sapamd_engine *e);
 float sapamd_last_force_ms(sapamd_engine *e);
 float sapamd_last_predict_ms(sapamd_engine *e);
+/* j-update scatter kernel time of the last call (timing on; 0 when the call had
+ * no updates) and, in *nupd (may be NULL), the number of updates it applied */
+float sapamd_last_scatter_ms(sapamd_engine *e, int *nupd);
 /* device time of the last force call from the first kernel after its input
  * copies to the end of the finalize kernel (copies excluded) */
 float sapamd_last_device_ms(sapamd_engine *e);

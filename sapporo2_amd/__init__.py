@@ -53,7 +53,7 @@ ENGINE_SYMBOLS = [
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
     "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing", "sapamd_set_graphs", "sapamd_graph_count", "sapamd_retrieve_predicted_j_particles",
     "sapamd_state_bytes", "sapamd_save_state", "sapamd_load_state",
-    "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
+    "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_scatter_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
     "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
     "sapamd_start_grav_calc_clusters", "sapamd_g6_engine", "sapamd_grav_calc_noalloc",
     "sapamd_noalloc_scratch_bytes",
@@ -110,6 +110,8 @@ def lib():
         L.sapamd_last_force_ms.restype = ctypes.c_float
         L.sapamd_last_predict_ms.argtypes = [_vp]
         L.sapamd_last_predict_ms.restype = ctypes.c_float
+        L.sapamd_last_scatter_ms.argtypes = [_vp, _ip]
+        L.sapamd_last_scatter_ms.restype = ctypes.c_float
         L.sapamd_last_config.argtypes = [_vp, _ip, _ip, _ip]
         L.sapamd_num_cus.argtypes = [_vp]
         L.sapamd_load_tuning.argtypes = [_vp, ctypes.c_char_p]
@@ -351,6 +353,12 @@ class Engine:
 
     def last_predict_ms(self):
         return float(self._L.sapamd_last_predict_ms(self._h))
+
+    def last_scatter_ms(self):
+        """(scatter kernel ms, updates applied) of the last call (timing on)."""
+        n = ctypes.c_int(0)
+        ms = float(self._L.sapamd_last_scatter_ms(self._h, ctypes.byref(n)))
+        return ms, n.value
 
     def last_config(self):
         P = ctypes.c_int(); S = ctypes.c_int(); jc = ctypes.c_int()

@@ -98,7 +98,8 @@ struct BlockIn {
 struct CallRec {
   int step, block, ni, P;
   double first_s, wait_s, read_s, copy_s;  // firsthalf, lasthalf2, read_neighbour_list, get_neighbour_list loop
-  float dev_ms, force_ms;
+  float dev_ms, force_ms, pred_ms, scat_ms;
+  int nupd;
 };
 
 double median(std::vector<double> v) {
@@ -255,6 +256,8 @@ int main(int argc, char** argv) {
           if (timing) {
             r.dev_ms = sapamd_last_device_ms(h);
             r.force_ms = sapamd_last_force_ms(h);
+            r.pred_ms = sapamd_last_predict_ms(h);
+            r.scat_ms = sapamd_last_scatter_ms(h, &r.nupd);
           }
           sapamd_last_config(h, &r.P, nullptr, nullptr);
           recs.push_back(r);
@@ -313,9 +316,18 @@ int main(int argc, char** argv) {
   for (auto& sel : w.blocks) inter_step += (double)sel.size() * w.n;
   std::vector<double> step_dev(steps, 0.0), step_f64(steps, 0.0), step_first(steps, 0.0), step_wait(steps, 0.0),
       step_read(steps, 0.0);
-  double f64_inter = 0, f64_ms = 0;
-  long f64_launches = 0;
+  double f64_inter = 0, f64_ms = 0, pred_ms = 0, scat_ms = 0, scat_upd = 0;
+  long f64_launches = 0, pred_launches = 0, scat_launches = 0;
   for (const CallRec& r : recs) {
+    if (r.pred_ms > 0) {
+      pred_ms += r.pred_ms;
+      ++pred_launches;
+    }
+    if (r.nupd > 0) {
+      scat_ms += r.scat_ms;
+      scat_upd += r.nupd;
+      ++scat_launches;
+    }
     step_dev[r.step] += r.dev_ms;
     step_first[r.step] += r.first_s;
     step_wait[r.step] += r.wait_s;
@@ -343,6 +355,9 @@ int main(int argc, char** argv) {
   json_array(o, "step_force64_ms", step_f64);
   fprintf(o, "\"force64\": {\"launches\": %ld, \"interactions\": %.17g, \"ms\": %.9g}, ", f64_launches, f64_inter,
           f64_ms);
+  fprintf(o, "\"predict\": {\"launches\": %ld, \"ms\": %.9g, \"nj\": %d}, ", pred_launches, pred_ms, w.n);
+  fprintf(o, "\"scatter\": {\"launches\": %ld, \"ms\": %.9g, \"updates\": %.17g}, ", scat_launches, scat_ms,
+          scat_upd);
   fprintf(o, "\"blocks\": [");
   for (int b = 0; b < w.nblocks; ++b) {
     std::vector<double> api(steps, 0.0), dev(steps, 0.0), fst(steps, 0.0), wt(steps, 0.0), rd(steps, 0.0),

@@ -130,6 +130,13 @@ int sapamd_graph_count(sapamd_engine* e) {
 }
 float sapamd_last_force_ms(sapamd_engine* e) { return e ? e->eng.last_force_ms() : 0.f; }
 float sapamd_last_predict_ms(sapamd_engine* e) { return e ? e->eng.last_predict_ms() : 0.f; }
+float sapamd_last_scatter_ms(sapamd_engine* e, int* nupd) {
+  if (!e) {
+    if (nupd) *nupd = 0;
+    return 0.f;
+  }
+  return e->eng.last_scatter_ms(nupd);
+}
 float sapamd_last_device_ms(sapamd_engine* e) { return e ? e->eng.last_device_ms() : 0.f; }
 int sapamd_last_config(sapamd_engine* e, int* P, int* S, int* jchunk) {
   if (!e) return -1;

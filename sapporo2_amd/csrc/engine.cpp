@@ -50,14 +50,22 @@ hipError_t hmalloc(T** p, size_t n) {
   if (n == 0) n = 1;
   return hipHostMalloc(reinterpret_cast<void**>(p), n * sizeof(T), hipHostMallocDefault);
 }
-// Pinned host memory mapped into the device's address space, coherent: the
-// kernels write results into it directly (host pointer h, device alias d).
+// Pinned host memory mapped into the device's address space: the kernels
+// write results into it directly (host pointer h, device alias d).  Coherent
+// (fine-grained) memory is written through to the host during the kernel;
+// non-coherent memory is cached in the GPU's L2 and written back by the
+// system-scope release of the call's completion event.  Choice:
+// $SAPPORO2_AMD_HOST_COHERENT (default shown in host_coherent()).
+bool host_coherent() {
+  const char* v = getenv("SAPPORO2_AMD_HOST_COHERENT");
+  return v ? atoi(v) != 0 : true;
+}
 template <typename T>
 hipError_t hmap(T** h, T** d, size_t n) {
   *h = *d = nullptr;
   if (n == 0) n = 1;
-  const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(h), n * sizeof(T),
-                                     hipHostMallocMapped | hipHostMallocCoherent);
+  const unsigned flags = hipHostMallocMapped | (host_coherent() ? hipHostMallocCoherent : hipHostMallocNonCoherent);
+  const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(h), n * sizeof(T), flags);
   if (e != hipSuccess) return e;
   return hipHostGetDevicePointer(reinterpret_cast<void**>(d), *h, 0);
 }
@@ -129,6 +137,8 @@ int Engine::open(int device, int order, int precision) {
   HIPCHK(hipEventCreate(&ev_f1_), "hipEventCreate");
   HIPCHK(hipEventCreate(&ev_p0_), "hipEventCreate");
   HIPCHK(hipEventCreate(&ev_p1_), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_s0_), "hipEventCreate");
+  HIPCHK(hipEventCreate(&ev_s1_), "hipEventCreate");
   for (auto& s : stage_) HIPCHK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(hipEventCreateWithFlags(&ev_desc_, hipEventDisableTiming), "hipEventCreate");
   HIPCHK(dmalloc(&pack_ctr_d_, 1), "pack counter");
@@ -191,7 +201,7 @@ int Engine::close() {
   dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
   dfree(ens_order_d_); dfree(ens_ctr_d_); ens_order_cap_ = 0;
   icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
-  for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_}) {
+  for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_, &ev_s0_, &ev_s1_}) {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
   }
@@ -644,6 +654,7 @@ FinalizeArgs Engine::finalize_args(int grid_ni, bool ngb_on) const {
   fz.packed = packed_hd_;
   fz.packed_cap = packed_cap_;
   fz.pack_ctr = pack_ctr_d_;
+  fz.slots = 1;
   fz.ngb_on = ngb_on ? 1 : 0;
   return fz;
 }
@@ -840,7 +851,8 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ea.ngb_cnt = cnt_d_;
   ea.ngb_list = list_d_;
   ea.ngb_on = ngb_on ? 1 : 0;
-  const FinalizeArgs fz = finalize_args(ni, ngb_on);
+  FinalizeArgs fz = finalize_args(ni, ngb_on);
+  fz.slots = 0;  // ensembles: packed lists (one slot of NGB_CAP per i-particle would not fit)
   if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
   if (launch_force_ens(stream_, order_, f64_, ea, fz, n64, (long)cus_ * 4 * 8) != 0)
     return fail("ensemble force launch", hipGetLastError());
@@ -855,6 +867,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   have_results_ = true;
   ngb_fetched_ = false;
   lists_in_slots_ = false;
+  lists_packed_ = true;
   ngb_on_last_ = ngb_on;
   ni_last_ = ni;
   nj_last_ = cl_total_;
@@ -935,6 +948,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   have_results_ = true;
   ngb_fetched_ = false;
   lists_in_slots_ = false;
+  lists_packed_ = false;
   ngb_on_last_ = ngb_on;
   ni_last_ = ni;
   nj_last_ = nj;
@@ -958,7 +972,14 @@ int Engine::enqueue_call(int grid_ni, int nj, const ForceConfig& cfg, bool ngb_o
     a.cd = desc_d_;
     a.grid_blocks = sblocks;
     a.pred_j1 = predict ? nj : 0;
+    const bool ev = timing_ && !capturing_;
+    if (ev) HIPCHK(hipEventRecord(ev_s0_, stream_), "event");
     if (launch_scatter(stream_, order_, f64_, a) != 0) return fail("scatter launch", hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev_s1_, stream_), "event");
+    timed_scatter_ = ev;
+    last_nupd_ = desc_h_->nupd;
+  } else {
+    timed_scatter_ = false;
   }
   if (predict && predict_range(0, nj) != 0) return -1;
   ForceArgs fa{};
@@ -1158,6 +1179,15 @@ float Engine::last_device_ms() {
   return ms;
 }
 
+float Engine::last_scatter_ms(int* nupd) {
+  if (nupd) *nupd = timed_scatter_ ? last_nupd_ : 0;
+  if (!timing_ || !timed_scatter_) return 0.f;
+  (void)hipEventSynchronize(ev_s1_);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ev_s0_, ev_s1_) != hipSuccess) return 0.f;
+  return ms;
+}
+
 float Engine::last_predict_ms() {
   if (!timing_ || !timed_predict_) return 0.f;
   (void)hipEventSynchronize(ev_p1_);
@@ -1205,9 +1235,10 @@ int Engine::get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3],
 
 // Reference read_ngb_list (sapporohostclass.cpp:804-835): the lists of the
 // last call; returns 1 if any i-particle has >= 256 neighbours.  The finalize
-// wrote them sorted and packed into host-mapped memory, so once the call is
-// complete nothing is copied, unless their total exceeded the packed buffer:
-// then the sorted device slots are copied back instead.
+// wrote them sorted into host-mapped memory (a slot per i-particle, or packed
+// for ensembles), so once the call is complete nothing is copied, unless an
+// ensemble's packed total exceeded its buffer: then the sorted device slots
+// are copied back instead.
 int Engine::read_ngb_list() {
   if (!open_ || !have_results_) return 0;
   if (wait_results() != 0) return -1;
@@ -1220,7 +1251,7 @@ int Engine::read_ngb_list() {
     overflow |= c >= NGB_CAP;
     total += std::min(c, NGB_CAP);
   }
-  if (ngb_on_last_ && total > packed_cap_ && !lists_in_slots_) {
+  if (ngb_on_last_ && lists_packed_ && total > packed_cap_ && !lists_in_slots_) {
     slot_lists_h_.resize((size_t)ni * NGB_CAP);
     HIPCHK(hipMemcpyAsync(slot_lists_h_.data(), list_d_, sizeof(int) * (size_t)ni * NGB_CAP, hipMemcpyDeviceToHost,
                           stream_), "list slot download");
@@ -1246,8 +1277,19 @@ int Engine::get_ngb_list(int ipipe, int maxlen, int* n, int* list) {
   const int c = std::min(len, NGB_CAP);
   const int ncopy = std::max(0, std::min(c, maxlen));
   if (list && ncopy > 0) {
-    const int* src = lists_in_slots_ ? slot_lists_h_.data() + (size_t)ipipe * NGB_CAP : packed_h_ + loff_h_[ipipe];
+    const int* src = lists_in_slots_  ? slot_lists_h_.data() + (size_t)ipipe * NGB_CAP
+                     : lists_packed_ ? packed_h_ + loff_h_[ipipe]
+                                     : packed_h_ + (size_t)ipipe * NGB_CAP;
     memcpy(list, src, sizeof(int) * (size_t)ncopy);
+  }
+  // callers read the lists in i order: start fetching a later one (the GPU wrote
+  // them to host memory, so each first touch misses the CPU caches)
+  const int ahead = ipipe + 8;
+  if (ahead < ni_last_ && !lists_packed_ && !lists_in_slots_) {
+    const char* p = reinterpret_cast<const char*>(packed_h_ + (size_t)ahead * NGB_CAP);
+    __builtin_prefetch(p);
+    __builtin_prefetch(p + 64);
+    __builtin_prefetch(result_field(part_fields(order_) + 2) + ahead);
   }
   if (n) *n = len;
   const bool overflow = len >= maxlen;

@@ -111,6 +111,8 @@ class Engine {
   int graph_count() const { return (int)graph_cache_.size(); }
   float last_force_ms();          // force kernel time of the last call (timing on)
   float last_predict_ms();        // predictor time of the last call (timing on, 0 if none)
+  // scatter (j-update) kernel time of the last call and its update count (timing on, 0 if none)
+  float last_scatter_ms(int* nupd);
   float last_device_ms();         // scatter..finalize (lists packed) device time of the last call (timing on)
   sapk::ForceConfig last_config() const { return last_cfg_; }
   int device() const { return device_; }
@@ -189,8 +191,12 @@ class Engine {
   int* pack_ctr_d_ = nullptr;
   int* cnt_d_ = nullptr; int* list_d_ = nullptr; int list_cap_ = 0;
   std::vector<int> slot_lists_h_;   // device list slots read back when the packed lists did not fit
-  bool lists_in_slots_ = false;
+  bool lists_in_slots_ = false;     // lists read back from the device slots (packed overflow)
+  bool lists_packed_ = false;       // last call's host lists are packed (ensembles), else one slot per i
   hipEvent_t ev_done_ = nullptr, ev_d0_ = nullptr, ev_d1_ = nullptr, ev_f0_ = nullptr, ev_f1_ = nullptr, ev_p0_ = nullptr, ev_p1_ = nullptr;
+  hipEvent_t ev_s0_ = nullptr, ev_s1_ = nullptr;
+  bool timed_scatter_ = false;
+  int last_nupd_ = 0;
   bool inflight_ = false, have_results_ = false, ngb_fetched_ = false, ngb_on_last_ = false;
   bool timed_predict_ = false, timed_force_ = false;
   bool capturing_ = false;

@@ -133,9 +133,11 @@ inline int partial_rows(const ForceConfig& c) { return (c.S % 4 == 0 && !c.smajo
 
 // Results and neighbour lists go straight to host-mapped pinned memory (no
 // copy command): field-major result rows out[f * ostride + i], and each sorted
-// list at packed[list_off[i] ...], its offset taken from pack_ctr (one atomic
-// per work-group).  A list that would end beyond packed_cap is not written to
-// the host; every sorted list is also kept in ngb_list (device) for that case.
+// list either in its own host slot packed[i * NGB_CAP ...] (slots = 1: a fixed
+// stride the host reads in order) or packed at packed[list_off[i] ...], its
+// offset taken from pack_ctr (one atomic per work-group; for ensembles, whose
+// slots would not fit).  A packed list that would end beyond packed_cap is not
+// written to the host; every sorted list is also kept in ngb_list (device).
 struct FinalizeArgs {
   const double* part; const double* part_ds2; const int* part_addr;
   int S;
@@ -150,6 +152,7 @@ struct FinalizeArgs {
   int* packed;           // host-mapped: sorted ids of all lists, packed_cap ints
   long packed_cap;
   int* pack_ctr;         // device: next free slot of packed (zeroed by the scatter)
+  int slots;             // 1: list i at packed[i * NGB_CAP]; 0: packed with list_off
   int ngb_on;
 };
 

@@ -721,8 +721,8 @@ __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
 }
 
 // Writes the sorted list s[0..c) of i-particle i: back into its device slots
-// and, when it fits, to the host-mapped packed buffer at off (consecutive lanes
-// write consecutive ids).
+// and, when it fits, to the host-mapped buffer at off (consecutive lanes write
+// consecutive ids).
 __device__ __forceinline__ void store_list(const FinalizeArgs& a, int i, const int* s, int c, int off, int t,
                                            int nt) {
   const bool host = (long)off + c <= a.packed_cap;
@@ -734,8 +734,8 @@ __device__ __forceinline__ void store_list(const FinalizeArgs& a, int i, const i
 }
 
 // One wave per i-particle: ascending sort of its (at most NGB_CAP) list in LDS,
-// then packing.  The four lists of a work-group take their packed offsets with
-// one atomic on the call's counter.
+// then the host copy: in the i-particle's slot, or packed, the four lists of a
+// work-group taking their offsets with one atomic on the call's counter.
 __device__ __forceinline__ void sort_lists_block(const FinalizeArgs& a, int b) {
   __shared__ int lst[4][NGB_CAP];
   __shared__ int srt[4][NGB_CAP];
@@ -749,15 +749,18 @@ __device__ __forceinline__ void sort_lists_block(const FinalizeArgs& a, int b) {
   for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
   if (lane == 0) cnts[w] = c;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && !a.slots) {
     const int tot = cnts[0] + cnts[1] + cnts[2] + cnts[3];
     base = tot > 0 ? atomicAdd(a.pack_ctr, tot) : 0;
   }
   wave_rank_sort(lst[w], c, lane, srt[w]);
   __syncthreads();
-  int off = base;
-  for (int u = 0; u < w; ++u) off += cnts[u];
-  if (valid && lane == 0) a.list_off[i] = off;
+  int off = i * NGB_CAP;
+  if (!a.slots) {
+    off = base;
+    for (int u = 0; u < w; ++u) off += cnts[u];
+    if (valid && lane == 0) a.list_off[i] = off;
+  }
   store_list(a, i, srt[w], c, off, lane, 64);
 }
 
@@ -823,7 +826,9 @@ __global__ __launch_bounds__(256) void k_finalize_block(FinalizeArgs a) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) R[f] = shT[f][0];
     write_row<ORDER, F64>(a, i, R, shD[0], shA[0], cnt);
-    if (a.ngb_on) {
+    if (a.ngb_on && a.slots) {
+      base = i * NGB_CAP;
+    } else if (a.ngb_on) {
       base = c > 0 ? atomicAdd(a.pack_ctr, c) : 0;
       a.list_off[i] = base;
     }

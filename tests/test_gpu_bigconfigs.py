@@ -9,6 +9,11 @@
 * The 512-cluster ensemble (N log-uniform in [256, 16384], seed 4, eps2 = 1e-4, ~2.2 M
   particles) in one call with uneven active sets: two i-particles of every cluster against
   the oracle, with lists.
+* Plummer 2^20 + 1 % central mass (eps2 = 1e-8, t_j on 28 power-of-two levels down to
+  2^-30, j-particles predicted to the block time), the whole system active in calls of
+  16384: the predicted j-state bit-exact against the oracle predictor, forces of 512 random
+  i-particles, the central mass and its 63 nearest stars within tolerance, their nearest
+  neighbours and lists bit-exact.
 * A lattice with many exact distance ties and ids decreasing with address, over lane shapes
   P in {1, 4, 16, 64} and several j-splits: nearest neighbour and lists bit-exact (ties go to
   the smallest j address, oracle/oracle.c; the reference's first-in-iteration rule,
@@ -159,6 +164,55 @@ def test_ensemble_512_clusters_full_size(prec):
                               f"cluster {c} prec={prec}")
         check_forces({k: np.concatenate(v) for k, v in got.items()}, {k: np.concatenate(v) for k, v in refs.items()},
                      1, TOL[prec], label=f"512 clusters prec={prec}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_central_mass_2e20_deep_levels_all_active(prec):
+    nstar, eps2 = 1048576, 1e-8
+    _say(f"central mass 2^20, 28 levels, all active, prec={prec}")
+    m, x, v = ics.plummer_bh(nstar, seed=91, mbh=0.01)
+    n = nstar + 1
+    ids = np.arange(n, dtype=np.int32)
+    h2 = np.concatenate([ics.neighbour_h2(x[:nstar], 32.0, mtot=0.99), [1e-4]])
+    rng = np.random.Generator(np.random.PCG64(94))
+    # Taylor coefficients of the right scale (the predictor's arithmetic does not depend on
+    # their being the system's own derivatives)
+    acc = rng.normal(0.0, 1.0, (n, 3))
+    jrk = rng.normal(0.0, 10.0, (n, 3))
+    lev = ics.block_levels(n, 28, seed=93)
+    deep = rng.choice(n, 1024, replace=False)       # the geometric occupancy leaves the
+    lev[deep] = 20 + np.arange(1024) % 8            # deepest levels nearly empty: fill them
+    tj = -rng.uniform(0.0, 1.0, n) * 2.0 ** -(3.0 + lev)
+    pj, vj, _ = oracle.predict(1, 0.0, tj, x, v, acc, jrk)
+    rad2 = np.sum(x * x, axis=1)
+    smp = np.unique(np.concatenate([rng.choice(n, 512, replace=False), np.argsort(rad2)[:64]]))
+    got = {k: np.zeros((len(smp), 3)) for k in ("acc", "jrk")}
+    got.update({k: np.zeros(len(smp)) for k in ("pot", "dsmin")})
+    got["nnb"] = np.zeros(len(smp), np.int32)
+    lists = [None] * len(smp)
+    with sa.Engine(sa.FOURTH, prec) as eng:
+        eng.set_j_particles(np.arange(n), ids, m, x, v, acc, jrk, tj=tj)
+        eng.set_time(0.0)
+        for c0 in range(0, n, sa.NPIPES):
+            ch = np.arange(c0, min(n, c0 + sa.NPIPES))
+            eng.start_grav_calc(n, ids[ch], pj[ch], vj[ch], eps2, h2[ch])
+            r = eng.get_grav_results(n, len(ch))
+            eng.read_ngb_list()
+            pos = np.nonzero((smp >= c0) & (smp < c0 + len(ch)))[0]
+            for k in got:
+                got[k][pos] = r[k][smp[pos] - c0]
+            for q in pos:
+                lists[q] = eng.get_ngb_list(int(smp[q] - c0))[:2]
+        probe = np.unique(np.concatenate([deep[:256], rng.choice(n, 128, replace=False)]))
+        rp = eng.retrieve_predicted_j_particles(probe)
+        assert np.array_equal(rp["pos"], pj[probe]) and np.array_equal(rp["vel"], vj[probe]), "predicted j-state"
+    _say(f"  {len(smp)} sampled i-particles against the oracle")
+    ref = oracle.force(1, pj[smp], vj[smp], ids[smp], pj, vj, m, ids, eps2, h2=h2[smp])
+    label = f"central mass 2^20 all active prec={prec}"
+    check_forces(got, ref, 1, TOL[prec], label=label)
+    nn_addr, nn_ds2, cnt, lst = _expected_nn(prec == sa.DOUBLE, pj[smp], ids[smp], pj, vj, m, ids, h2[smp], eps2,
+                                             ref)
+    _compare_nn_lists(got["nnb"], got["dsmin"], lists, nn_addr, nn_ds2, cnt, lst, ids, label)
 
 
 @pytest.mark.parametrize("prec", PRECS)

@@ -128,6 +128,9 @@ float sapamd_last_predict_ms(sapamd_engine *e);
 /* j-update scatter kernel time of the last call (timing on; 0 when the call had
  * no updates) and, in *nupd (may be NULL), the number of updates it applied */
 float sapamd_last_scatter_ms(sapamd_engine *e, int *nupd);
+/* finalize kernel time of the last call (timing on): reduction of the partial
+ * sums, list sort, and the result and list writes into host memory */
+float sapamd_last_finalize_ms(sapamd_engine *e);
 /* device time of the last force call from the first kernel after its input
  * copies to the end of the finalize kernel (copies excluded) */
 float sapamd_last_device_ms(sapamd_engine *e);

@@ -53,7 +53,7 @@ ENGINE_SYMBOLS = [
     "sapamd_force_prediction", "sapamd_retrieve_predicted_j_particle",
     "sapamd_retrieve_j_particle_state", "sapamd_set_force_override", "sapamd_set_split_major", "sapamd_set_ngb_pipeline", "sapamd_set_timing", "sapamd_set_graphs", "sapamd_graph_count", "sapamd_retrieve_predicted_j_particles",
     "sapamd_state_bytes", "sapamd_save_state", "sapamd_load_state",
-    "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_scatter_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
+    "sapamd_last_force_ms", "sapamd_last_predict_ms", "sapamd_last_scatter_ms", "sapamd_last_finalize_ms", "sapamd_last_config", "sapamd_load_tuning", "sapamd_tuning_entries", "sapamd_num_cus",
     "sapamd_synchronize", "sapamd_last_error", "sapamd_set_clusters", "sapamd_cluster_offset",
     "sapamd_start_grav_calc_clusters", "sapamd_g6_engine", "sapamd_grav_calc_noalloc",
     "sapamd_noalloc_scratch_bytes",
@@ -112,6 +112,8 @@ def lib():
         L.sapamd_last_predict_ms.restype = ctypes.c_float
         L.sapamd_last_scatter_ms.argtypes = [_vp, _ip]
         L.sapamd_last_scatter_ms.restype = ctypes.c_float
+        L.sapamd_last_finalize_ms.argtypes = [_vp]
+        L.sapamd_last_finalize_ms.restype = ctypes.c_float
         L.sapamd_last_config.argtypes = [_vp, _ip, _ip, _ip]
         L.sapamd_num_cus.argtypes = [_vp]
         L.sapamd_load_tuning.argtypes = [_vp, ctypes.c_char_p]
@@ -228,12 +230,30 @@ class Engine:
             self._h, int(nj), ni, _pi(ids_), _pd(xi_), _pd(vi_), _pd(ai_), float(eps2), _pd(h2_)),
             "start_grav_calc")
 
-    def get_grav_results(self, nj, ni, ngb=True):
-        """Returns dict acc, jrk, snp (SIXTH), pot, nnb, dsmin (NaN where not written)."""
-        acc = np.zeros((ni, 3)); jrk = np.zeros((ni, 3)); pot = np.zeros(ni)
-        snp = np.zeros((ni, 3)) if self.order >= SIXTH else None
-        crk = np.zeros((ni, 3)) if self.order >= SIXTH else None
-        nnb = np.zeros(ni, np.int32); dsmin = np.full(ni, np.nan)
+    def get_grav_results(self, nj, ni, ngb=True, out=None):
+        """Returns dict acc, jrk, snp (SIXTH), pot, nnb, dsmin (NaN where not written).
+        out: a dict of that form from an earlier call, with arrays of at least ni rows,
+        whose first ni rows are filled and returned instead of new arrays."""
+        if out is None:
+            acc = np.zeros((ni, 3)); jrk = np.zeros((ni, 3)); pot = np.zeros(ni)
+            snp = np.zeros((ni, 3)) if self.order >= SIXTH else None
+            crk = np.zeros((ni, 3)) if self.order >= SIXTH else None
+            nnb = np.zeros(ni, np.int32); dsmin = np.full(ni, np.nan)
+        else:
+            def rows(k, shape, dtype):
+                a = out.get(k)
+                if a is None or a.dtype != dtype or a.shape[1:] != shape or len(a) < ni or not a.flags.c_contiguous:
+                    raise ValueError(f"get_grav_results: out[{k!r}] must be a contiguous {dtype} array of "
+                                     f">= {ni} rows of shape {shape}")
+                return a[:ni]
+            acc = rows("acc", (3,), np.float64); jrk = rows("jrk", (3,), np.float64)
+            pot = rows("pot", (), np.float64)
+            snp = rows("snp", (3,), np.float64) if self.order >= SIXTH else None
+            crk = rows("crk", (3,), np.float64) if self.order >= SIXTH else None
+            nnb = rows("nnb", (), np.int32); dsmin = rows("dsmin", (), np.float64)
+            dsmin.fill(np.nan)
+            if not ngb:
+                nnb.fill(0)
         self._check(self._L.sapamd_get_grav_results(
             self._h, int(nj), int(ni), _pd(acc), _pd(jrk), _pd(snp), _pd(crk), _pd(pot), _pi(nnb),
             _pd(dsmin), 1 if ngb else 0), "get_grav_results")
@@ -353,6 +373,9 @@ class Engine:
 
     def last_predict_ms(self):
         return float(self._L.sapamd_last_predict_ms(self._h))
+
+    def last_finalize_ms(self):
+        return float(self._L.sapamd_last_finalize_ms(self._h))
 
     def last_scatter_ms(self):
         """(scatter kernel ms, updates applied) of the last call (timing on)."""

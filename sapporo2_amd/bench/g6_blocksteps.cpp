@@ -98,7 +98,7 @@ struct BlockIn {
 struct CallRec {
   int step, block, ni, P;
   double first_s, wait_s, read_s, copy_s;  // firsthalf, lasthalf2, read_neighbour_list, get_neighbour_list loop
-  float dev_ms, force_ms, pred_ms, scat_ms;
+  float dev_ms, force_ms, pred_ms, scat_ms, fin_ms;
   int nupd;
 };
 
@@ -258,6 +258,7 @@ int main(int argc, char** argv) {
             r.force_ms = sapamd_last_force_ms(h);
             r.pred_ms = sapamd_last_predict_ms(h);
             r.scat_ms = sapamd_last_scatter_ms(h, &r.nupd);
+            r.fin_ms = sapamd_last_finalize_ms(h);
           }
           sapamd_last_config(h, &r.P, nullptr, nullptr);
           recs.push_back(r);
@@ -361,13 +362,14 @@ int main(int argc, char** argv) {
   fprintf(o, "\"blocks\": [");
   for (int b = 0; b < w.nblocks; ++b) {
     std::vector<double> api(steps, 0.0), dev(steps, 0.0), fst(steps, 0.0), wt(steps, 0.0), rd(steps, 0.0),
-        fo(steps, 0.0);
+        fo(steps, 0.0), fz(steps, 0.0);
     int calls = 0, P = 0;
     for (const CallRec& r : recs) {
       if (r.block != b) continue;
       api[r.step] += r.first_s + r.wait_s + r.read_s;
       dev[r.step] += r.dev_ms;
       fo[r.step] += r.force_ms;
+      fz[r.step] += r.fin_ms;
       fst[r.step] += r.first_s;
       wt[r.step] += r.wait_s;
       rd[r.step] += r.read_s;
@@ -376,8 +378,8 @@ int main(int argc, char** argv) {
     }
     fprintf(o,
             "%s{\"ni\": %zu, \"calls\": %d, \"P\": %d, \"api_ms\": %.6g, \"device_ms\": %.6g, \"force_ms\": %.6g, "
-            "\"firsthalf_ms\": %.6g, \"lasthalf_ms\": %.6g, \"read_ngb_ms\": %.6g}",
-            b ? ", " : "", w.blocks[b].size(), calls, P, 1e3 * median(api), median(dev), median(fo),
+            "\"finalize_ms\": %.6g, \"firsthalf_ms\": %.6g, \"lasthalf_ms\": %.6g, \"read_ngb_ms\": %.6g}",
+            b ? ", " : "", w.blocks[b].size(), calls, P, 1e3 * median(api), median(dev), median(fo), median(fz),
             1e3 * median(fst), 1e3 * median(wt), 1e3 * median(rd));
   }
   fprintf(o, "]}\n");

@@ -137,6 +137,7 @@ float sapamd_last_scatter_ms(sapamd_engine* e, int* nupd) {
   }
   return e->eng.last_scatter_ms(nupd);
 }
+float sapamd_last_finalize_ms(sapamd_engine* e) { return e ? e->eng.last_finalize_ms() : 0.f; }
 float sapamd_last_device_ms(sapamd_engine* e) { return e ? e->eng.last_device_ms() : 0.f; }
 int sapamd_last_config(sapamd_engine* e, int* P, int* S, int* jchunk) {
   if (!e) return -1;

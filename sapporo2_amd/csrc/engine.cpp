@@ -5,10 +5,14 @@
 #include <immintrin.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <system_error>
+#include <thread>
+#include <vector>
 
 namespace sap {
 
@@ -37,6 +41,36 @@ constexpr size_t CALL_HDR = 256;
 inline size_t call_ids_bytes(int bucket) { return ((size_t)bucket * sizeof(int) + 255) / 256 * 256; }
 constexpr size_t CALL_DOUBLES = (CALL_HDR + ((size_t)NPIPES * sizeof(int) + 255) / 256 * 256) / sizeof(double) +
                                 (size_t)IBUF_FIELDS * NPIPES;
+
+// Host-side copies of very large calls (ensembles carry millions of
+// i-particles per call) are split over threads: fn(begin, end) on [0, n) in
+// chunks of at least HOST_GRAIN elements, at most HOST_THREADS threads, the
+// calling thread taking the first chunk.  Smaller ranges run inline.
+constexpr size_t HOST_GRAIN = 65536;
+constexpr unsigned HOST_THREADS = 16;
+template <typename F>
+void par_for(size_t n, F&& fn) {
+  const unsigned hw = std::max(1u, std::min(HOST_THREADS, std::thread::hardware_concurrency()));
+  const size_t T = std::min<size_t>(hw, n / HOST_GRAIN);
+  if (T <= 1) {
+    fn((size_t)0, n);
+    return;
+  }
+  const size_t chunk = (n + T - 1) / T;
+  std::vector<std::thread> th;
+  th.reserve(T - 1);
+  for (size_t t = 1; t < T; ++t) {
+    const size_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    try {
+      th.emplace_back([&fn, b, e] { fn(b, e); });
+    } catch (const std::system_error&) {
+      fn(b, e);  // no thread available: this chunk runs here
+    }
+  }
+  fn((size_t)0, std::min(n, chunk));
+  for (std::thread& x : th) x.join();
+}
 
 template <typename T>
 hipError_t dmalloc(T** p, size_t n) {
@@ -694,24 +728,29 @@ int Engine::cluster_offset(int c) const {
 // any h2 > 0 (neighbour search on).
 bool Engine::pack_i(int ni, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
                     const double* h2, double* bx, double* bv, double* ba, double* bh, int* bid) {
-  const size_t n3 = 3 * (size_t)ni;
-  memcpy(bx, xi, sizeof(double) * n3);
-  if (vi) memcpy(bv, vi, sizeof(double) * n3);
-  else memset(bv, 0, sizeof(double) * n3);
-  if (order_ >= ORDER_H6) {
-    if (ai) memcpy(ba, ai, sizeof(double) * n3);
-    else memset(ba, 0, sizeof(double) * n3);
-  }
-  bool ngb_on = false;
-  if (h2) {
-    memcpy(bh, h2, sizeof(double) * (size_t)ni);
-    for (int i = 0; i < ni; ++i) ngb_on |= h2[i] > 0.0;
-  } else {
-    memset(bh, 0, sizeof(double) * (size_t)ni);
-  }
-  if (id) memcpy(bid, id, sizeof(int) * (size_t)ni);
-  else std::fill(bid, bid + ni, -1);
-  return ngb_on;
+  const bool h6 = order_ >= ORDER_H6;
+  std::atomic<bool> ngb_on{false};
+  par_for((size_t)ni, [&](size_t b, size_t e) {
+    const size_t n = e - b;
+    memcpy(bx + 3 * b, xi[b], sizeof(double) * 3 * n);
+    if (vi) memcpy(bv + 3 * b, vi[b], sizeof(double) * 3 * n);
+    else memset(bv + 3 * b, 0, sizeof(double) * 3 * n);
+    if (h6) {
+      if (ai) memcpy(ba + 3 * b, ai[b], sizeof(double) * 3 * n);
+      else memset(ba + 3 * b, 0, sizeof(double) * 3 * n);
+    }
+    if (h2) {
+      memcpy(bh + b, h2 + b, sizeof(double) * n);
+      bool any = false;
+      for (size_t i = b; i < e; ++i) any |= h2[i] > 0.0;
+      if (any) ngb_on.store(true, std::memory_order_relaxed);
+    } else {
+      memset(bh + b, 0, sizeof(double) * n);
+    }
+    if (id) memcpy(bid + b, id + b, sizeof(int) * n);
+    else std::fill(bid + b, bid + e, -1);
+  });
+  return ngb_on.load();
 }
 
 int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, const double (*xi)[3],
@@ -1179,6 +1218,14 @@ float Engine::last_device_ms() {
   return ms;
 }
 
+float Engine::last_finalize_ms() {
+  if (!timing_ || !have_results_ || !timed_force_) return 0.f;
+  (void)hipEventSynchronize(ev_d1_);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ev_f1_, ev_d1_) != hipSuccess) return 0.f;
+  return ms;
+}
+
 float Engine::last_scatter_ms(int* nupd) {
   if (nupd) *nupd = timed_scatter_ ? last_nupd_ : 0;
   if (!timing_ || !timed_scatter_) return 0.f;
@@ -1207,29 +1254,32 @@ int Engine::get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3],
   if (wait_results() != 0) return -1;
   const int NF = part_fields(order_);
   // result rows are field-major in the host-mapped block
-  auto vec3 = [&](double (*dst)[3], int f0) {
-    if (!dst) return;
-    const double *x = result_field(f0), *y = result_field(f0 + 1), *z = result_field(f0 + 2);
-    for (int i = 0; i < ni; ++i) { dst[i][0] = x[i]; dst[i][1] = y[i]; dst[i][2] = z[i]; }
-  };
-  vec3(acc, 0);
-  if (pot) memcpy(pot, result_field(3), sizeof(double) * (size_t)ni);
-  vec3(jrk, 4);
-  if (order_ >= ORDER_H6) {
-    vec3(snp, 7);
-    if (crk)
-      for (int i = 0; i < ni; ++i) crk[i][0] = crk[i][1] = crk[i][2] = 0.0;
-  }
-  if (ngb) {
-    const double *ds = result_field(NF), *nn = result_field(NF + 1);
-    for (int i = 0; i < ni; ++i) {
-      if (nnb) nnb[i] = 0;
-      if (ds[i] < 1.0e10) {
-        if (nnb) nnb[i] = (int)nn[i];
-        if (dsmin) dsmin[i] = ds[i];
+  const bool h6 = order_ >= ORDER_H6;
+  par_for((size_t)ni, [&](size_t b, size_t e) {
+    auto vec3 = [&](double (*dst)[3], int f0) {
+      if (!dst) return;
+      const double *x = result_field(f0), *y = result_field(f0 + 1), *z = result_field(f0 + 2);
+      for (size_t i = b; i < e; ++i) { dst[i][0] = x[i]; dst[i][1] = y[i]; dst[i][2] = z[i]; }
+    };
+    vec3(acc, 0);
+    if (pot) memcpy(pot + b, result_field(3) + b, sizeof(double) * (e - b));
+    vec3(jrk, 4);
+    if (h6) {
+      vec3(snp, 7);
+      if (crk)
+        for (size_t i = b; i < e; ++i) crk[i][0] = crk[i][1] = crk[i][2] = 0.0;
+    }
+    if (ngb) {
+      const double *ds = result_field(NF), *nn = result_field(NF + 1);
+      for (size_t i = b; i < e; ++i) {
+        if (nnb) nnb[i] = 0;
+        if (ds[i] < 1.0e10) {
+          if (nnb) nnb[i] = (int)nn[i];
+          if (dsmin) dsmin[i] = ds[i];
+        }
       }
     }
-  }
+  });
   return 0;
 }
 

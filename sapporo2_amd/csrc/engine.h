@@ -113,6 +113,7 @@ class Engine {
   float last_predict_ms();        // predictor time of the last call (timing on, 0 if none)
   // scatter (j-update) kernel time of the last call and its update count (timing on, 0 if none)
   float last_scatter_ms(int* nupd);
+  float last_finalize_ms();       // finalize (reduction, list sort, host writes) time of the last call (timing on)
   float last_device_ms();         // scatter..finalize (lists packed) device time of the last call (timing on)
   sapk::ForceConfig last_config() const { return last_cfg_; }
   int device() const { return device_; }

@@ -143,6 +143,13 @@ def test_ensemble_512_clusters_full_size(prec):
         _say(f"ensemble call: {len(I)} i-particles, {int(sizes.sum())} j-particles")
         eng.start_grav_calc_clusters(cidx, I, X, V, eps, H)
         r = eng.get_grav_results(1, len(I))
+        # the same results into caller-kept arrays longer than the call, holding stale values
+        ni = len(I)
+        junk = dict(acc=np.full((ni + 99, 3), 7.0), jrk=np.full((ni + 99, 3), 7.0), pot=np.full(ni + 99, 7.0),
+                    nnb=np.full(ni + 99, 7, np.int32), dsmin=np.full(ni + 99, 7.0))
+        r2 = eng.get_grav_results(1, ni, out=junk)
+        for k in ("acc", "jrk", "pot", "nnb", "dsmin"):
+            assert np.array_equal(r2[k], r[k], equal_nan=k == "dsmin"), f"out= rows differ: {k}"
         eng.read_ngb_list()
         starts = np.concatenate([[0], np.cumsum([len(p) for p in pick])[:-1]])
         got = {k: [] for k in ("acc", "jrk", "pot")}

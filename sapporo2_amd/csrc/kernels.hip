@@ -195,18 +195,34 @@ struct IState {
   int id, i;
 };
 
+template <typename R> __device__ __forceinline__ R minR(R a, R b);
+template <> __device__ __forceinline__ float minR<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> __device__ __forceinline__ double minR<double>(double a, double b) { return __builtin_fmin(a, b); }
+
+// Relative position and squared distance of one pair, the arithmetic every
+// distance test uses (the nearest-neighbour resolution recomputes it and must
+// agree bit for bit).
+template <typename R, typename Rec, typename St>
+__device__ __forceinline__ R pair_ds2(const Rec& r, const St& s, R& dx, R& dy, R& dz) {
+  dx = to_rel(r.x, s.x);
+  dy = to_rel(r.y, s.y);
+  dz = to_rel(r.z, s.z);
+  return fmaR<R>(dz, dz, fmaR<R>(dy, dy, dx * dx));
+}
+
 constexpr int NF4 = 7, NF6 = 10;
 template <int ORDER> constexpr int nf() { return ORDER >= ORDER_H6 ? NF6 : NF4; }
 
-// One pair interaction (kernels.cu:565-667) accumulated into t[].
-template <int ORDER, bool NGB, bool CHECK, typename R, typename Rec>
+// One pair interaction (kernels.cu:565-667) accumulated into t[].  Nearest
+// neighbour: with BLOCKMIN only the minimum distance is kept (dsmin is then
+// the running minimum of a sub-block, whose address the caller resolves),
+// otherwise the distance and the address j of the nearest so far.
+template <int ORDER, bool NGB, bool CHECK, bool BLOCKMIN, typename R, typename Rec>
 __device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, const IState<ORDER, R>& s,
                                      const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
                                      int* __restrict__ ngb_cnt, int* __restrict__ ngb_list) {
-  const R dx = to_rel(r.x, s.x);
-  const R dy = to_rel(r.y, s.y);
-  const R dz = to_rel(r.z, s.z);
-  const R ds2 = fmaR<R>(dz, dz, fmaR<R>(dy, dy, dx * dx));
+  R dx, dy, dz;
+  const R ds2 = pair_ds2<R>(r, s, dx, dy, dz);
   // bitwise & keeps the predicates branch-free (no exec-masked regions)
   const bool use = CHECK ? (ok & (r.id != s.id)) : ok;
   const R inv0 = rsq<R>(ds2 + eps2);
@@ -253,9 +269,13 @@ __device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, c
     t[6] = t[6] + jz;
   }
   // nearest neighbour: strict <, first address wins inside a lane (kernels.cu:539-546)
-  const bool better = use & (ds2 < dsmin);
-  dsmin = better ? ds2 : dsmin;
-  nnaddr = better ? j : nnaddr;
+  if constexpr (BLOCKMIN) {
+    dsmin = minR<R>(dsmin, CHECK ? (use ? ds2 : R(INFINITY)) : ds2);
+  } else {
+    const bool better = use & (ds2 < dsmin);
+    dsmin = better ? ds2 : dsmin;
+    nnaddr = better ? j : nnaddr;
+  }
   // neighbour predicate h2_i > ds2 (kernels.cu:584); the caller appends
   return NGB & use & (s.h2 > ds2);
 }
@@ -287,10 +307,10 @@ __device__ __forceinline__ void ngb_flush(PendingNgb& p, int* __restrict__ ngb_l
 
 // Empty register constraint: the values are computed before this point.
 template <int ORDER, typename R>
-__device__ __forceinline__ void settle(R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr) {
+__device__ __forceinline__ void settle(R (&t)[nf<ORDER>()], R& smin) {
 #pragma unroll
   for (int f = 0; f < nf<ORDER>(); ++f) asm volatile("" : "+v"(t[f]));
-  asm volatile("" : "+v"(dsmin), "+v"(nnaddr));
+  asm volatile("" : "+v"(smin));
 }
 
 // Four consecutive j of one lane with a single neighbour branch for the group.
@@ -298,20 +318,20 @@ __device__ __forceinline__ void settle(R (&t)[nf<ORDER>()], R& dsmin, int& nnadd
 // sinks it below the (rarely taken) append block and keeps the four relative
 // positions live across it, which costs a wave of occupancy.
 template <int ORDER, bool NGB, bool CHECK, int Q, bool PIPE, typename R, typename Rec>
-__device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const int k0,
-                                      const IState<ORDER, R>& s, const R eps2, R (&t)[nf<ORDER>()], R& dsmin,
-                                      int& nnk, int* __restrict__ ngb_cnt, int* __restrict__ ngb_list,
-                                      PendingNgb& pend) {
+__device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const IState<ORDER, R>& s,
+                                      const R eps2, R (&t)[nf<ORDER>()], R& smin, int* __restrict__ ngb_cnt,
+                                      int* __restrict__ ngb_list, PendingNgb& pend) {
   bool nb[4];
   int idj[4];
+  int unused = 0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int j = j0 + u * Q;
     const Rec& r = rec[j];
     idj[u] = r.id;
-    nb[u] = pair<ORDER, NGB, CHECK>(r, k0 + u, true, s, eps2, t, dsmin, nnk, ngb_cnt, ngb_list);
+    nb[u] = pair<ORDER, NGB, CHECK, true>(r, j, true, s, eps2, t, smin, unused, ngb_cnt, ngb_list);
   }
-  if constexpr (NGB) settle<ORDER>(t, dsmin, nnk);
+  if constexpr (NGB) settle<ORDER>(t, smin);
   // without lists, keep the scheduler from hoisting the scalar loads of a whole
   // unrolled sub-block (they exhaust the SGPRs and spill)
   if constexpr (!NGB) __builtin_amdgcn_sched_barrier(0);
@@ -375,7 +395,10 @@ __device__ __forceinline__ void wave_force(
 #pragma unroll
   for (int f = 0; f < NF; ++f) T[f] = 0.0;
   R dsmin = R(INFINITY);
-  nnaddr = INT_MAX;
+  // nearest pair so far: -1 none; >= 0 the first address of the sub-block of
+  // 16 pairs (whole tiles) that holds it; <= -2 the exact address a, as -2 - a
+  // (the partial tile at the end of the range)
+  int nnblk = -1;
   PendingNgb pend;
 #pragma unroll
   for (int u = 0; u < 4; ++u) { pend.m[u] = false; pend.k[u] = 0; pend.id[u] = 0; }
@@ -403,37 +426,42 @@ __device__ __forceinline__ void wave_force(
           check = __ballot(st.id >= r.x && st.id <= r.y) != 0;
         }
       }
-      // The nearest-neighbour candidate is recorded as its index within a
-      // sub-block of 16 pairs (a compile-time inline constant: no register
-      // copy of the j index per pair) and turned into an address per sub-block.
+      // Nearest neighbour: each pair only lowers the minimum of its sub-block of
+      // 16 pairs (one v_min); a sub-block that beats the lane's nearest so far
+      // records its first address, resolved to the pair after the loop.
       if (check) {
         for (int k = 0; k < KT; k += 16) {
-          int nnk = -1;
+          R sm = R(INFINITY);
 #pragma unroll
           for (int q = 0; q < 16; q += 4)
-            pair4<ORDER, NGB, true, Q, PIPE>(rec, jt + (k + q) * Q, q, st, eps2, t, dsmin, nnk, ngb_cnt, ngb_list,
-                                             pend);
-          nnaddr = nnk >= 0 ? jt + (k + nnk) * Q : nnaddr;
+            pair4<ORDER, NGB, true, Q, PIPE>(rec, jt + (k + q) * Q, st, eps2, t, sm, ngb_cnt, ngb_list, pend);
+          const bool better = sm < dsmin;
+          dsmin = better ? sm : dsmin;
+          nnblk = better ? jt + k * Q : nnblk;
         }
       } else {
         for (int k = 0; k < KT; k += 16) {
-          int nnk = -1;
+          R sm = R(INFINITY);
 #pragma unroll
           for (int q = 0; q < 16; q += 4)
-            pair4<ORDER, NGB, false, Q, PIPE>(rec, jt + (k + q) * Q, q, st, eps2, t, dsmin, nnk, ngb_cnt, ngb_list,
-                                              pend);
-          nnaddr = nnk >= 0 ? jt + (k + nnk) * Q : nnaddr;
+            pair4<ORDER, NGB, false, Q, PIPE>(rec, jt + (k + q) * Q, st, eps2, t, sm, ngb_cnt, ngb_list, pend);
+          const bool better = sm < dsmin;
+          dsmin = better ? sm : dsmin;
+          nnblk = better ? jt + k * Q : nnblk;
         }
       }
     } else {
       if constexpr (NGB && PIPE) ngb_flush(pend, ngb_list, st.i);
+      const R before = dsmin;
+      int cand = 0;
       for (int k = 0; k < kn; ++k) {
         const int j = jt + k * Q;
         const bool ok = j < je;
         const Rec& r = rec[ok ? j : jb];
-        if (pair<ORDER, NGB, true>(r, j, ok, st, eps2, t, dsmin, nnaddr, ngb_cnt, ngb_list))
+        if (pair<ORDER, NGB, true, false>(r, j, ok, st, eps2, t, dsmin, cand, ngb_cnt, ngb_list))
           ngb_append(ngb_cnt, ngb_list, st.i, r.id);
       }
+      if (dsmin < before) nnblk = -2 - cand;  // a later pair, strictly nearer: its address is exact
     }
     if constexpr (!F64) {
 #pragma unroll
@@ -443,6 +471,22 @@ __device__ __forceinline__ void wave_force(
   if constexpr (F64) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) T[f] = (double)t[f];
+  }
+  // The nearest pair is the first of its sub-block at distance dsmin: a
+  // sub-block replaced the candidate only when strictly nearer, so this is the
+  // lane's first address at the minimum, as the per-pair rule gives.
+  nnaddr = nnblk <= -2 ? -2 - nnblk : INT_MAX;
+  if (nnblk >= 0) {
+    bool found = false;
+    for (int u = 0; u < 16; ++u) {
+      const int j = nnblk + u * Q;
+      const Rec& r = rec[j];
+      R dx, dy, dz;
+      const R d2 = pair_ds2<R>(r, st, dx, dy, dz);
+      const bool hit = !found && (r.id != st.id) && d2 == dsmin;
+      nnaddr = hit ? j : nnaddr;
+      found = found || hit;
+    }
   }
 
   if constexpr (NGB && PIPE) ngb_flush(pend, ngb_list, st.i);

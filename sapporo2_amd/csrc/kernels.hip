@@ -214,9 +214,9 @@ constexpr int NF4 = 7, NF6 = 10;
 template <int ORDER> constexpr int nf() { return ORDER >= ORDER_H6 ? NF6 : NF4; }
 
 // One pair interaction (kernels.cu:565-667) accumulated into t[].  Nearest
-// neighbour: with BLOCKMIN only the minimum distance is kept (dsmin is then
-// the running minimum of a sub-block, whose address the caller resolves),
-// otherwise the distance and the address j of the nearest so far.
+// neighbour: with BLOCKMIN, dsmin receives the pair's squared distance (+inf
+// for an excluded pair) for the caller's group minimum; otherwise dsmin and
+// nnaddr hold the distance and the address j of the nearest so far.
 template <int ORDER, bool NGB, bool CHECK, bool BLOCKMIN, typename R, typename Rec>
 __device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, const IState<ORDER, R>& s,
                                      const R eps2, R (&t)[nf<ORDER>()], R& dsmin, int& nnaddr,
@@ -270,7 +270,7 @@ __device__ __forceinline__ bool pair(const Rec& r, const int j, const bool ok, c
   }
   // nearest neighbour: strict <, first address wins inside a lane (kernels.cu:539-546)
   if constexpr (BLOCKMIN) {
-    dsmin = minR<R>(dsmin, CHECK ? (use ? ds2 : R(INFINITY)) : ds2);
+    dsmin = CHECK ? (use ? ds2 : R(INFINITY)) : ds2;
   } else {
     const bool better = use & (ds2 < dsmin);
     dsmin = better ? ds2 : dsmin;
@@ -307,10 +307,9 @@ __device__ __forceinline__ void ngb_flush(PendingNgb& p, int* __restrict__ ngb_l
 
 // Empty register constraint: the values are computed before this point.
 template <int ORDER, typename R>
-__device__ __forceinline__ void settle(R (&t)[nf<ORDER>()], R& smin) {
+__device__ __forceinline__ void settle(R (&t)[nf<ORDER>()]) {
 #pragma unroll
   for (int f = 0; f < nf<ORDER>(); ++f) asm volatile("" : "+v"(t[f]));
-  asm volatile("" : "+v"(smin));
 }
 
 // Four consecutive j of one lane with a single neighbour branch for the group.
@@ -321,7 +320,7 @@ template <int ORDER, bool NGB, bool CHECK, int Q, bool PIPE, typename R, typenam
 __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0, const IState<ORDER, R>& s,
                                       const R eps2, R (&t)[nf<ORDER>()], R& smin, int* __restrict__ ngb_cnt,
                                       int* __restrict__ ngb_list, PendingNgb& pend) {
-  bool nb[4];
+  R d[4];
   int idj[4];
   int unused = 0;
 #pragma unroll
@@ -329,30 +328,33 @@ __device__ __forceinline__ void pair4(const Rec* __restrict__ rec, const int j0,
     const int j = j0 + u * Q;
     const Rec& r = rec[j];
     idj[u] = r.id;
-    nb[u] = pair<ORDER, NGB, CHECK, true>(r, j, true, s, eps2, t, smin, unused, ngb_cnt, ngb_list);
+    pair<ORDER, NGB, CHECK, true>(r, j, true, s, eps2, t, d[u], unused, ngb_cnt, ngb_list);
   }
-  if constexpr (NGB) settle<ORDER>(t, smin);
+  // the group's nearest distance: it lowers the sub-block minimum, and h2 > it
+  // is the test whether any of the four pairs is a neighbour
+  const R g = minR<R>(minR<R>(minR<R>(d[0], d[1]), d[2]), d[3]);
+  smin = minR<R>(smin, g);
+  if constexpr (NGB) settle<ORDER>(t);
   // without lists, keep the scheduler from hoisting the scalar loads of a whole
   // unrolled sub-block (they exhaust the SGPRs and spill)
   if constexpr (!NGB) __builtin_amdgcn_sched_barrier(0);
-  // the group test is a wave-uniform OR of the four compare masks (a bool OR
-  // reduction would be vectorised into per-lane bit arithmetic)
-  const bool any = NGB && (__ballot(nb[0]) | __ballot(nb[1]) | __ballot(nb[2]) | __ballot(nb[3])) != 0;
+  const bool any = NGB && __ballot(s.h2 > g) != 0;
   if constexpr (NGB && PIPE) {
     ngb_flush(pend, ngb_list, s.i);
     if (any) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (nb[u]) pend.k[u] = atomicAdd(&ngb_cnt[s.i], 1);
+        const bool nb = s.h2 > d[u];
+        if (nb) pend.k[u] = atomicAdd(&ngb_cnt[s.i], 1);
         pend.id[u] = idj[u];
-        pend.m[u] = nb[u];
+        pend.m[u] = nb;
       }
     }
   } else if constexpr (NGB) {
     if (any) {
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (nb[u]) ngb_append(ngb_cnt, ngb_list, s.i, idj[u]);
+        if (s.h2 > d[u]) ngb_append(ngb_cnt, ngb_list, s.i, idj[u]);
     }
   }
 }

@@ -688,7 +688,7 @@ FinalizeArgs Engine::finalize_args(int grid_ni, bool ngb_on) const {
   fz.packed = packed_hd_;
   fz.packed_cap = packed_cap_;
   fz.pack_ctr = pack_ctr_d_;
-  fz.slots = 1;
+  fz.inorder = 1;
   fz.ngb_on = ngb_on ? 1 : 0;
   return fz;
 }
@@ -891,7 +891,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ea.ngb_list = list_d_;
   ea.ngb_on = ngb_on ? 1 : 0;
   FinalizeArgs fz = finalize_args(ni, ngb_on);
-  fz.slots = 0;  // ensembles: packed lists (one slot of NGB_CAP per i-particle would not fit)
+  fz.inorder = 0;  // ensembles: packed lists (NGB_CAP per i-particle would not fit)
   if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
   if (launch_force_ens(stream_, order_, f64_, ea, fz, n64, (long)cus_ * 4 * 8) != 0)
     return fail("ensemble force launch", hipGetLastError());
@@ -1327,19 +1327,8 @@ int Engine::get_ngb_list(int ipipe, int maxlen, int* n, int* list) {
   const int c = std::min(len, NGB_CAP);
   const int ncopy = std::max(0, std::min(c, maxlen));
   if (list && ncopy > 0) {
-    const int* src = lists_in_slots_  ? slot_lists_h_.data() + (size_t)ipipe * NGB_CAP
-                     : lists_packed_ ? packed_h_ + loff_h_[ipipe]
-                                     : packed_h_ + (size_t)ipipe * NGB_CAP;
+    const int* src = lists_in_slots_ ? slot_lists_h_.data() + (size_t)ipipe * NGB_CAP : packed_h_ + loff_h_[ipipe];
     memcpy(list, src, sizeof(int) * (size_t)ncopy);
-  }
-  // callers read the lists in i order: start fetching a later one (the GPU wrote
-  // them to host memory, so each first touch misses the CPU caches)
-  const int ahead = ipipe + 8;
-  if (ahead < ni_last_ && !lists_packed_ && !lists_in_slots_) {
-    const char* p = reinterpret_cast<const char*>(packed_h_ + (size_t)ahead * NGB_CAP);
-    __builtin_prefetch(p);
-    __builtin_prefetch(p + 64);
-    __builtin_prefetch(result_field(part_fields(order_) + 2) + ahead);
   }
   if (n) *n = len;
   const bool overflow = len >= maxlen;

@@ -193,7 +193,7 @@ class Engine {
   int* cnt_d_ = nullptr; int* list_d_ = nullptr; int list_cap_ = 0;
   std::vector<int> slot_lists_h_;   // device list slots read back when the packed lists did not fit
   bool lists_in_slots_ = false;     // lists read back from the device slots (packed overflow)
-  bool lists_packed_ = false;       // last call's host lists are packed (ensembles), else one slot per i
+  bool lists_packed_ = false;       // last call's host lists are atomically packed (ensembles), else in i order
   hipEvent_t ev_done_ = nullptr, ev_d0_ = nullptr, ev_d1_ = nullptr, ev_f0_ = nullptr, ev_f1_ = nullptr, ev_p0_ = nullptr, ev_p1_ = nullptr;
   hipEvent_t ev_s0_ = nullptr, ev_s1_ = nullptr;
   bool timed_scatter_ = false;

@@ -133,11 +133,15 @@ inline int partial_rows(const ForceConfig& c) { return (c.S % 4 == 0 && !c.smajo
 
 // Results and neighbour lists go straight to host-mapped pinned memory (no
 // copy command): field-major result rows out[f * ostride + i], and each sorted
-// list either in its own host slot packed[i * NGB_CAP ...] (slots = 1: a fixed
-// stride the host reads in order) or packed at packed[list_off[i] ...], its
-// offset taken from pack_ctr (one atomic per work-group; for ensembles, whose
-// slots would not fit).  A packed list that would end beyond packed_cap is not
-// written to the host; every sorted list is also kept in ngb_list (device).
+// list at packed[list_off[i] ...].  Single-system calls (inorder = 1) pack the
+// lists in i order within regions of LIST_REGION i-particles, region r from
+// packed[r * LIST_REGION * NGB_CAP], so the host reads them as one stream; an
+// offset is its region's base plus the stored lengths before it there.
+// Ensembles (inorder = 0, more i-particles than a region layout would fit)
+// take offsets from pack_ctr, one atomic per work-group.  A packed list that
+// would end beyond packed_cap is not written to the host; every sorted list is
+// also kept in ngb_list (device).
+constexpr int LIST_REGION = 256;
 struct FinalizeArgs {
   const double* part; const double* part_ds2; const int* part_addr;
   int S;
@@ -152,7 +156,7 @@ struct FinalizeArgs {
   int* packed;           // host-mapped: sorted ids of all lists, packed_cap ints
   long packed_cap;
   int* pack_ctr;         // device: next free slot of packed (zeroed by the scatter)
-  int slots;             // 1: list i at packed[i * NGB_CAP]; 0: packed with list_off
+  int inorder;           // 1: i-ordered regions (single system); 0: atomic offsets (ensembles)
   int ngb_on;
 };
 

@@ -766,6 +766,19 @@ __global__ __launch_bounds__(256) void k_finalize_rows(FinalizeArgs a) {
   finalize_rows_block<ORDER, F64>(a, blockIdx.x);
 }
 
+// Host offset of i-particle i's list in a single-system call: its region's
+// base plus the stored lengths of the region's earlier lists.  The counts are
+// final once the force kernel has run, so no work-group waits for another; a
+// whole wave calls this (at most LIST_REGION - 1 counts, a few per lane).
+__device__ __forceinline__ int region_offset(const int* __restrict__ cnt, int i, int lane) {
+  const int r0 = i & ~(LIST_REGION - 1);
+  int s = 0;
+  for (int k = r0 + lane; k < i; k += 64) s += min(cnt[k], NGB_CAP);
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m);
+  return r0 * NGB_CAP + s;
+}
+
 // Writes the sorted list s[0..c) of i-particle i: back into its device slots
 // and, when it fits, to the host-mapped buffer at off (consecutive lanes write
 // consecutive ids).
@@ -780,8 +793,9 @@ __device__ __forceinline__ void store_list(const FinalizeArgs& a, int i, const i
 }
 
 // One wave per i-particle: ascending sort of its (at most NGB_CAP) list in LDS,
-// then the host copy: in the i-particle's slot, or packed, the four lists of a
-// work-group taking their offsets with one atomic on the call's counter.
+// then the host copy: in i order within its region, or (ensembles) packed, the
+// four lists of a work-group taking their offsets with one atomic on the
+// call's counter.
 __device__ __forceinline__ void sort_lists_block(const FinalizeArgs& a, int b) {
   __shared__ int lst[4][NGB_CAP];
   __shared__ int srt[4][NGB_CAP];
@@ -795,18 +809,18 @@ __device__ __forceinline__ void sort_lists_block(const FinalizeArgs& a, int b) {
   for (int k = lane; k < c; k += 64) lst[w][k] = a.ngb_list[(size_t)i * NGB_CAP + k];
   if (lane == 0) cnts[w] = c;
   __syncthreads();
-  if (threadIdx.x == 0 && !a.slots) {
+  if (threadIdx.x == 0 && !a.inorder) {
     const int tot = cnts[0] + cnts[1] + cnts[2] + cnts[3];
     base = tot > 0 ? atomicAdd(a.pack_ctr, tot) : 0;
   }
   wave_rank_sort(lst[w], c, lane, srt[w]);
+  int off = valid && a.inorder ? region_offset(a.ngb_cnt, i, lane) : 0;
   __syncthreads();
-  int off = i * NGB_CAP;
-  if (!a.slots) {
+  if (!a.inorder) {
     off = base;
     for (int u = 0; u < w; ++u) off += cnts[u];
-    if (valid && lane == 0) a.list_off[i] = off;
   }
+  if (valid && lane == 0) a.list_off[i] = off;
   store_list(a, i, srt[w], c, off, lane, 64);
 }
 
@@ -872,11 +886,16 @@ __global__ __launch_bounds__(256) void k_finalize_block(FinalizeArgs a) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) R[f] = shT[f][0];
     write_row<ORDER, F64>(a, i, R, shD[0], shA[0], cnt);
-    if (a.ngb_on && a.slots) {
-      base = i * NGB_CAP;
-    } else if (a.ngb_on) {
+    if (a.ngb_on && !a.inorder) {
       base = c > 0 ? atomicAdd(a.pack_ctr, c) : 0;
       a.list_off[i] = base;
+    }
+  }
+  if (a.ngb_on && a.inorder && t < 64) {  // uniform over the block; a whole wave
+    const int o = region_offset(a.ngb_cnt, i, t);
+    if (t == 0) {
+      base = o;
+      a.list_off[i] = o;
     }
   }
   if (c == 0) return;  // uniform over the block

@@ -15,15 +15,16 @@ value       = interactions per step / median over the timed steps of the end-to-
               (g6_set_ti_ + g6calc_firsthalf_ -> g6calc_lasthalf2_ + g6_read_neighbour_list_
               for every call of the step; PCIe, host packing and result/list read-back
               included), the reference's convention (test_performance_blockStep.cpp:160-183);
-              the engine runs as a user runs it, without per-call HIP events.
+              the engine runs as a user runs it, without per-call HIP events.  The run
+              times twice the steps, alternating between that and the engine with its HIP
+              events on its stream around every call and kernel.
 device_value= the same interactions over the summed device windows of the calls (first kernel
-              after the call's input copy -> last kernel of the call), secondary; from a
-              second run of the same workload directly after, with the engine's HIP events
-              on its stream around every call and kernel (instrumented_value = that run's
-              API rate: the events cost the host ~16 us per call).
+              after the call's input copy -> last kernel of the call), secondary; median of
+              the instrumented steps (instrumented_value = their API rate: the events cost
+              the host ~16 us per call).
 roofline    = dominant kernel (force, P = 64 lane shape): 60 flop per interaction (reference
               convention) over its summed launch times, HIP events on its stream, in the
-              instrumented run.
+              instrumented steps of the same run.
 cpu_baseline= the FP64 direct summation of the oracle (OpenMP, compiled for this host with
               -march=native) on a sample of 16384 i-particles against all j, on this host's
               cores; the same sample is the accuracy reference for the reported errors.
@@ -193,20 +194,19 @@ def run_driver(wl, res, prec, steps, warmup, sample_block, timing=1):
 
 
 def measure(wl, res, prec, steps, warmup, sample_block):
-    """Two runs of the compiled caller on the same workload: the first as a user runs the
-    library (no HIP-event instrumentation) gives the API rate; the second, with the engine's
-    per-call events on, gives the device windows and the per-kernel times."""
-    plain = run_driver(wl, res, prec, steps, warmup, sample_block, timing=0)
-    timed = run_driver(wl, res, prec, steps, warmup, sample_block, timing=1)
-    return summarize(plain, timed, prec)
+    """One run of the compiled caller with 2 x steps timed steps alternating between the
+    engine as a user runs it (no HIP-event instrumentation: the API rate) and the engine with
+    its per-call events on (device windows, per-kernel times), so that both see the same
+    process and GPU state."""
+    return summarize(run_driver(wl, res, prec, steps, warmup, sample_block, timing=2), prec)
 
 
-def summarize(plain, js, prec):
-    """value (API, median step) from the uninstrumented run; device window, roofline of the
-    P = 64 force kernel and the per-block times from the instrumented run."""
-    ips = plain["interactions_per_step"]
-    api = float(np.median(plain["step_api_s"]))
-    api_timed = float(np.median(js["step_api_s"]))
+def summarize(js, prec):
+    """value (API, median of the uninstrumented steps); device window, roofline of the P = 64
+    force kernel and the per-block device times from the instrumented steps."""
+    ips = js["interactions_per_step"]
+    api = float(np.median(js["step_api_s"]))
+    api_timed = float(np.median(js["step_api_instrumented_s"]))
     dev_ms = float(np.median(js["step_device_ms"]))
     f = js["force64"]
     achieved = FLOP_PER_INTERACTION * f["interactions"] / (f["ms"] * 1e-3) / 1e12 if f["ms"] > 0 else 0.0
@@ -226,11 +226,10 @@ def summarize(plain, js, prec):
                 device_value=ips / (dev_ms * 1e-3) / 1e9 if dev_ms > 0 else None, device_ms_per_step=dev_ms,
                 force_mean_ms=f["ms"] / max(1, f["launches"]), force_launches=f["launches"],
                 achieved_tflops=achieved, peak=PEAK_TFLOPS[prec],
-                host_ms_per_step={k: round(1e3 * float(np.median(plain[k])), 3) for k in
+                host_ms_per_step={k: round(1e3 * float(np.median(js[k])), 3) for k in
                                   ("step_setti_s", "step_firsthalf_s", "step_lasthalf_s", "step_read_ngb_s")},
-                outside_api_ms_per_step={"j_resend": round(1e3 * float(np.median(plain["step_resend_s"])), 3),
-                                         "list_copy_out": round(1e3 * float(np.median(plain["step_list_copy_s"])),
-                                                                3)},
+                outside_api_ms_per_step={"j_resend": round(1e3 * float(np.median(js["step_resend_s"])), 3),
+                                         "list_copy_out": round(1e3 * float(np.median(js["step_list_copy_s"])), 3)},
                 blocks=js["blocks"])
 
 
@@ -620,9 +619,9 @@ def main():
                                f"eps2={EPS2}, neighbour lists (mean ~32, cap 256) + NN, through the g6 C-ABI",
                    "n": w.n, "eps2": EPS2, "active_counts": w.counts, "npipes": sa.NPIPES,
                    "kernel": main_name, "timed": "g6_set_ti_ + g6calc_firsthalf_ .. g6calc_lasthalf2_ + "
-                                                 "g6_read_neighbour_list_, median step of the timed steps, engine "
-                                                 "instrumentation off (device window and kernel times from a "
-                                                 "second, instrumented run)",
+                                                 "g6_read_neighbour_list_, median of the timed steps run with the "
+                                                 "engine's instrumentation off (device window and kernel times "
+                                                 "from instrumented steps interleaved with them)",
                    "driver": "sapporo2_amd/bench/g6_blocksteps"},
         "roofline": {"bound": "valu", "achieved": round(r["achieved_tflops"], 3),
                      "peak": r["peak"], "unit": "TFLOP/s",

@@ -15,7 +15,10 @@
 //                 predicted j-state, read back by bench.py for the accuracy checks
 //   sample_block  block index whose results are kept (default: the last block)
 //   timing        1 (default): the engine's HIP events report each call's device
-//                 window and force-kernel time; 0: off
+//                 window and kernel times; 0: off; 2: interleaved, 2 x steps timed
+//                 steps alternating off (even) and on (odd), so that the API rate
+//                 (from the uninstrumented steps) and the device times (from the
+//                 instrumented ones) come from the same process and GPU state
 // Prints one JSON line on stdout.
 #include <algorithm>
 #include <chrono>
@@ -97,6 +100,7 @@ struct BlockIn {
 // Per-call measurements.
 struct CallRec {
   int step, block, ni, P;
+  bool inst;  // the step ran with the engine's HIP events on
   double first_s, wait_s, read_s, copy_s;  // firsthalf, lasthalf2, read_neighbour_list, get_neighbour_list loop
   float dev_ms, force_ms, pred_ms, scat_ms, fin_ms;
   int nupd;
@@ -156,7 +160,8 @@ int main(int argc, char** argv) {
     return 1;
   }
   sapamd_engine* h = sapamd_g6_engine();
-  sapamd_set_timing(h, timing);
+  sapamd_set_timing(h, timing == 1);
+  const int total = timing == 2 ? 2 * steps : steps;  // timed steps
   const int npipes = g6_npipes_();
   int nj = w.n;
 
@@ -200,10 +205,13 @@ int main(int argc, char** argv) {
 
   std::vector<CallRec> recs;
   std::vector<double> step_api, step_resend, step_copy, step_setti;
+  std::vector<char> step_inst;
   const Clock::time_point run0 = Clock::now();
-  for (int step = 0; step < warmup + steps; ++step) {
+  for (int step = 0; step < warmup + total; ++step) {
     const bool timed = step >= warmup;
-    const bool keep = step == warmup + steps - 1;
+    const bool keep = step == warmup + total - 1;
+    const bool inst = timing == 1 || (timing == 2 && step % 2 == 1);
+    if (timing == 2) sapamd_set_timing(h, inst ? 1 : 0);
     double api = 0, resend = 0, copy = 0, setti = 0;
     for (int b = 0; b < w.nblocks; ++b) {
       const auto& sel = w.blocks[b];
@@ -251,9 +259,9 @@ int main(int argc, char** argv) {
         copy += secs(t3, t4);
         if (timed) {
           CallRec r{};
-          r.step = step - warmup; r.block = b; r.ni = ni;
+          r.step = step - warmup; r.block = b; r.ni = ni; r.inst = inst;
           r.first_s = secs(t0, t1); r.wait_s = secs(t1, t2); r.read_s = secs(t2, t3); r.copy_s = secs(t3, t4);
-          if (timing) {
+          if (inst) {
             r.dev_ms = sapamd_last_device_ms(h);
             r.force_ms = sapamd_last_force_ms(h);
             r.pred_ms = sapamd_last_predict_ms(h);
@@ -276,6 +284,7 @@ int main(int argc, char** argv) {
       step_resend.push_back(resend);
       step_copy.push_back(copy);
       step_setti.push_back(setti);
+      step_inst.push_back(inst ? 1 : 0);
     }
   }
   const double run_s = secs(run0, Clock::now());
@@ -312,14 +321,27 @@ int main(int argc, char** argv) {
     fclose(f);
   }
 
-  // summary: per step, and per block (medians over the timed steps)
+  // summary: per step, and per block (medians over the timed steps).  With
+  // interleaved timing the API figures come from the uninstrumented steps and
+  // the device and kernel times from the instrumented ones.
   double inter_step = 0;
   for (auto& sel : w.blocks) inter_step += (double)sel.size() * w.n;
-  std::vector<double> step_dev(steps, 0.0), step_f64(steps, 0.0), step_first(steps, 0.0), step_wait(steps, 0.0),
-      step_read(steps, 0.0);
+  auto of_kind = [&](int s, bool inst) { return timing != 2 || (step_inst[s] != 0) == inst; };
+  auto pick = [&](const std::vector<double>& v, bool inst) {
+    std::vector<double> out;
+    for (int s = 0; s < (int)v.size(); ++s)
+      if (of_kind(s, inst)) out.push_back(v[s]);
+    return out;
+  };
+  std::vector<double> step_dev(total, 0.0), step_f64(total, 0.0), step_first(total, 0.0), step_wait(total, 0.0),
+      step_read(total, 0.0);
   double f64_inter = 0, f64_ms = 0, pred_ms = 0, scat_ms = 0, scat_upd = 0;
   long f64_launches = 0, pred_launches = 0, scat_launches = 0;
   for (const CallRec& r : recs) {
+    step_first[r.step] += r.first_s;
+    step_wait[r.step] += r.wait_s;
+    step_read[r.step] += r.read_s;
+    if (!r.inst) continue;
     if (r.pred_ms > 0) {
       pred_ms += r.pred_ms;
       ++pred_launches;
@@ -330,9 +352,6 @@ int main(int argc, char** argv) {
       ++scat_launches;
     }
     step_dev[r.step] += r.dev_ms;
-    step_first[r.step] += r.first_s;
-    step_wait[r.step] += r.wait_s;
-    step_read[r.step] += r.read_s;
     if (r.P == 64) {
       step_f64[r.step] += r.force_ms;
       f64_inter += (double)r.ni * w.n;
@@ -344,16 +363,17 @@ int main(int argc, char** argv) {
   fprintf(o, "{\"driver\": \"g6_blocksteps\", \"n\": %d, \"nblocks\": %d, \"steps\": %d, \"warmup\": %d, ", w.n,
           w.nblocks, steps, warmup);
   fprintf(o, "\"timing\": %d, \"interactions_per_step\": %.17g, \"run_s\": %.6f, ", timing, inter_step, run_s);
-  fprintf(o, "\"api_median_s\": %.9g, ", median(step_api));
-  json_array(o, "step_api_s", step_api);
-  json_array(o, "step_setti_s", step_setti);
-  json_array(o, "step_resend_s", step_resend);
-  json_array(o, "step_list_copy_s", step_copy);
-  json_array(o, "step_firsthalf_s", step_first);
-  json_array(o, "step_lasthalf_s", step_wait);
-  json_array(o, "step_read_ngb_s", step_read);
-  json_array(o, "step_device_ms", step_dev);
-  json_array(o, "step_force64_ms", step_f64);
+  fprintf(o, "\"api_median_s\": %.9g, ", median(pick(step_api, false)));
+  json_array(o, "step_api_s", pick(step_api, false));
+  json_array(o, "step_api_instrumented_s", pick(step_api, true));
+  json_array(o, "step_setti_s", pick(step_setti, false));
+  json_array(o, "step_resend_s", pick(step_resend, false));
+  json_array(o, "step_list_copy_s", pick(step_copy, false));
+  json_array(o, "step_firsthalf_s", pick(step_first, false));
+  json_array(o, "step_lasthalf_s", pick(step_wait, false));
+  json_array(o, "step_read_ngb_s", pick(step_read, false));
+  json_array(o, "step_device_ms", pick(step_dev, true));
+  json_array(o, "step_force64_ms", pick(step_f64, true));
   fprintf(o, "\"force64\": {\"launches\": %ld, \"interactions\": %.17g, \"ms\": %.9g}, ", f64_launches, f64_inter,
           f64_ms);
   fprintf(o, "\"predict\": {\"launches\": %ld, \"ms\": %.9g, \"nj\": %d}, ", pred_launches, pred_ms, w.n);
@@ -361,8 +381,8 @@ int main(int argc, char** argv) {
           scat_upd);
   fprintf(o, "\"blocks\": [");
   for (int b = 0; b < w.nblocks; ++b) {
-    std::vector<double> api(steps, 0.0), dev(steps, 0.0), fst(steps, 0.0), wt(steps, 0.0), rd(steps, 0.0),
-        fo(steps, 0.0), fz(steps, 0.0);
+    std::vector<double> api(total, 0.0), dev(total, 0.0), fst(total, 0.0), wt(total, 0.0), rd(total, 0.0),
+        fo(total, 0.0), fz(total, 0.0);
     int calls = 0, P = 0;
     for (const CallRec& r : recs) {
       if (r.block != b) continue;
@@ -379,8 +399,9 @@ int main(int argc, char** argv) {
     fprintf(o,
             "%s{\"ni\": %zu, \"calls\": %d, \"P\": %d, \"api_ms\": %.6g, \"device_ms\": %.6g, \"force_ms\": %.6g, "
             "\"finalize_ms\": %.6g, \"firsthalf_ms\": %.6g, \"lasthalf_ms\": %.6g, \"read_ngb_ms\": %.6g}",
-            b ? ", " : "", w.blocks[b].size(), calls, P, 1e3 * median(api), median(dev), median(fo), median(fz),
-            1e3 * median(fst), 1e3 * median(wt), 1e3 * median(rd));
+            b ? ", " : "", w.blocks[b].size(), calls, P, 1e3 * median(pick(api, false)), median(pick(dev, true)),
+            median(pick(fo, true)), median(pick(fz, true)), 1e3 * median(pick(fst, false)),
+            1e3 * median(pick(wt, false)), 1e3 * median(pick(rd, false)));
   }
   fprintf(o, "]}\n");
   fflush(o);

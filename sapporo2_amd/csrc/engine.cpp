@@ -1285,8 +1285,9 @@ int Engine::get_grav_results(int nj, int ni, double (*acc)[3], double (*jrk)[3],
 
 // Reference read_ngb_list (sapporohostclass.cpp:804-835): the lists of the
 // last call; returns 1 if any i-particle has >= 256 neighbours.  The finalize
-// wrote them sorted into host-mapped memory (a slot per i-particle, or packed
-// for ensembles), so once the call is complete nothing is copied, unless an
+// wrote them sorted into host-mapped memory (in i order within regions of
+// LIST_REGION i-particles, or, for ensembles, packed by work-group), so once
+// the call is complete nothing is copied, unless an
 // ensemble's packed total exceeded its buffer: then the sorted device slots
 // are copied back instead.
 int Engine::read_ngb_list() {

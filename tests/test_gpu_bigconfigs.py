@@ -2,7 +2,8 @@
 
 * The block-step sweep of the metric (Plummer N = 131072, eps2 = 1e-6, t_j on 20 power-of-two
   levels, j-particles predicted to the block time, the previous active set re-sent before each
-  block step, active sets 2^0 .. 2^17 in calls of <= 16384): every bucket is checked on a
+  block step after the first, so that the particles not yet re-sent are predicted from their own
+  t_j, active sets 2^0 .. 2^17 in calls of <= 16384): every bucket is checked on a
   sample of <= 1024 i-particles against the oracle -- forces within tolerance, nearest
   neighbours and neighbour lists bit-exact -- and the device's predicted j-state is checked
   bit-exact against the oracle predictor.
@@ -84,9 +85,10 @@ def test_block_step_sweep_n131072(prec):
     with sa.Engine(sa.FOURTH, prec) as eng:
         eng.set_j_particles(np.arange(n), ids, m, x, v, acc, jrk, tj=tj)
         for b, sel in enumerate(blocks):
-            prev = blocks[b - 1]
-            eng.set_j_particles(prev, ids[prev], m[prev], px[prev], pv[prev], acc[prev], jrk[prev], tj=0.0)
-            xs[prev], vs[prev], ts[prev] = px[prev], pv[prev], 0.0
+            if b > 0:
+                prev = blocks[b - 1]
+                eng.set_j_particles(prev, ids[prev], m[prev], px[prev], pv[prev], acc[prev], jrk[prev], tj=0.0)
+                xs[prev], vs[prev], ts[prev] = px[prev], pv[prev], 0.0
             eng.set_time(0.0)
             pj, vj, _ = oracle.predict(1, 0.0, ts, xs, vs, acc, jrk)
             smp = np.sort(smp_rng.choice(len(sel), min(1024, len(sel)), replace=False))

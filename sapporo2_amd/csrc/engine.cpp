@@ -233,7 +233,7 @@ int Engine::close() {
   dfree(gather_d_); gather_cap_ = 0;
   dfree(gaddr_d_); gaddr_cap_ = 0;
   dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
-  dfree(ens_order_d_); dfree(ens_ctr_d_); ens_order_cap_ = 0;
+  dfree(ens_order_d_); ens_order_cap_ = 0;
   icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
   for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_, &ev_s0_, &ev_s1_}) {
     if (*e) (void)hipEventDestroy(*e);
@@ -820,7 +820,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
       for (int s = 0; s < S; ++s) {
         const int jb = off + (int)std::min<long>((long)s * jch, nj);
         const int je = off + (int)std::min<long>((long)(s + 1) * jch, nj);
-        dst.push_back(EnsItem{i0, cnt, jb, je, g.P, g.c, 0, 0});
+        dst.push_back(EnsItem{i0, cnt, jb, je, g.P, g.c, S, 0});
       }
       for (int k = 0; k < cnt; ++k) ens_idesc_[i0 + k] = int4{first, S, k, g.P == 64 ? 0 : 1};
     }
@@ -843,7 +843,6 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
     HIPCHK(dmalloc(&ens_items_d_, (size_t)ens_items_cap_), "ensemble work table");
     HIPCHK(dmalloc(&ens_order_d_, (size_t)ens_items_cap_), "ensemble work queue");
   }
-  if (!ens_ctr_d_) HIPCHK(dmalloc(&ens_ctr_d_, (size_t)2), "ensemble queue counters");
   if (ens_eps_cap_ < nc) {
     dfree(ens_eps_d_);
     ens_eps_cap_ = nc;
@@ -853,7 +852,6 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
          "work table upload");
   HIPCHK(hipMemcpyAsync(ens_order_d_, ens_order_.data(), sizeof(int) * nitems, hipMemcpyHostToDevice, stream_),
          "work queue upload");
-  HIPCHK(hipMemsetAsync(ens_ctr_d_, 0, 2 * sizeof(int), stream_), "queue counters");
   HIPCHK(hipMemcpyAsync(idesc_d_, ens_idesc_.data(), sizeof(int4) * ni, hipMemcpyHostToDevice, stream_),
          "descriptor upload");
   HIPCHK(hipMemcpyAsync(ens_eps_d_, eps2c, sizeof(double) * nc, hipMemcpyHostToDevice, stream_), "eps upload");
@@ -882,7 +880,6 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ea.items = ens_items_d_;
   ea.nitems = nitems;
   ea.order = ens_order_d_;
-  ea.ctr = ens_ctr_d_;
   ea.idesc = idesc_d_;
   ea.part = part_d_;
   ea.part_ds2 = pds2_d_;
@@ -893,7 +890,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   FinalizeArgs fz = finalize_args(ni, ngb_on);
   fz.inorder = 0;  // ensembles: packed lists (NGB_CAP per i-particle would not fit)
   if (timing_) HIPCHK(hipEventRecord(ev_f0_, stream_), "event");
-  if (launch_force_ens(stream_, order_, f64_, ea, fz, n64, (long)cus_ * 4 * 8) != 0)
+  if (launch_force_ens(stream_, order_, f64_, ea, fz, n64) != 0)
     return fail("ensemble force launch", hipGetLastError());
   if (timing_) {
     HIPCHK(hipEventRecord(ev_f1_, stream_), "event");

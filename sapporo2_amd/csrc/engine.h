@@ -244,7 +244,6 @@ class Engine {
   int4* idesc_d_ = nullptr;
   std::vector<int> ens_order_;
   int* ens_order_d_ = nullptr; int ens_order_cap_ = 0;
-  int* ens_ctr_d_ = nullptr;
   double* ens_eps_d_ = nullptr; int ens_eps_cap_ = 0;
 
   struct TuneEntry { int order, prec, ngb, li, lj, P, jlog; };

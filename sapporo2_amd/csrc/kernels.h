@@ -161,16 +161,16 @@ struct FinalizeArgs {
 };
 
 // Ensemble work item: i-particles [i0, i0 + icount) of one cluster (icount <=
-// P, P in {64, 16, 4, 1}) against that cluster's j in [jb, je).
-struct EnsItem { int i0, icount, jb, je, P, cluster, pad0, pad1; };
+// P, P in {64, 16, 4, 1}) against that cluster's j in [jb, je), one of the
+// nsplit items that cover the cluster's j for these i-particles.
+struct EnsItem { int i0, icount, jb, je, P, cluster, nsplit, pad; };
 
 struct EnsArgs {
   const void* rec; const int2* idr;
   const double* ipos; const double* ivel; const double* iacc; const double* ih2; const int* iid;
   const double* eps2;        // per cluster
   const EnsItem* items; int nitems;
-  const int* order;          // queue: item indices, longest first within each part
-  int* ctr;                  // 2 queue counters (P = 64 part, small-P part), zero at launch
+  const int* order;          // item indices, longest first within each part
   const int4* idesc;         // per i: first item, number of splits, i-slot
   double* part; double* part_ds2; int* part_addr;   // part_fields x nitems x 64, nitems x 64
   int* ngb_cnt; int* ngb_list; int ngb_on;
@@ -186,9 +186,10 @@ int launch_predict(hipStream_t s, int order, bool f64, const PredictArgs& a);
 int launch_force(hipStream_t s, int order, bool f64, const ForceArgs& a, const ForceConfig& c);
 int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a);
 // Ensemble force + finalize (+ list sort) over a work table whose first n64
-// items have lane shape P = 64; at most max_waves persistent waves per part.
-int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64,
-                     long max_waves);
+// items have lane shape P = 64; one single-wave work-group per item.  Without
+// neighbour lists, an item that covers all of its cluster's j (nsplit = 1)
+// writes its i-particles' result rows itself, and the finalize skips them.
+int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64);
 int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a);
 // Writes the call descriptor v to device memory cd with a one-thread kernel
 // (no copy command, nothing allocated).

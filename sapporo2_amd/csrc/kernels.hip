@@ -598,11 +598,15 @@ template <int ORDER, bool F64>
 __device__ __forceinline__ void write_row(const FinalizeArgs& a, int i, const double* T, double ds, int ad,
                                           int cnt);
 
-// Ensemble force: a persistent grid of waves walks a work table of items
-// (cluster, i-group, j-split) round-robin; each item carries its own lane shape.
+// Ensemble force: one wave per item of a work table (cluster, i-group,
+// j-split); each item carries its own lane shape.  Without lists, an item
+// that is its i-particles' only split writes their result rows (over PCIe,
+// while other items still compute) as the finalize would: the finalize's
+// sum over one split is 0.0 + T, and its nearest neighbour the item's own.
 template <int ORDER, bool F64, int P, bool NGB>
-__device__ __forceinline__ void ens_item(const EnsArgs& a, const typename RecOf<ORDER, F64>::T* __restrict__ rec,
-                                         const EnsItem& it, int item) {
+__device__ __forceinline__ void ens_item(const EnsArgs& a, const FinalizeArgs& fz,
+                                         const typename RecOf<ORDER, F64>::T* __restrict__ rec, const EnsItem& it,
+                                         int item) {
   constexpr int NF = nf<ORDER>();
   double T[NF];
   double dsd;
@@ -611,6 +615,14 @@ __device__ __forceinline__ void ens_item(const EnsArgs& a, const typename RecOf<
   wave_force<ORDER, F64, P, NGB>(rec, a.idr, a.ipos, a.ivel, a.iacc, a.ih2, a.iid,
                                  it.i0, it.icount, it.jb, it.je, a.eps2[it.cluster], a.ngb_cnt, a.ngb_list, T,
                                  dsd, nnaddr, islot, writer);
+  if (!NGB && it.nsplit == 1) {
+    if (writer) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) T[f] = 0.0 + T[f];
+      write_row<ORDER, F64>(fz, it.i0 + islot, T, dsd, nnaddr, 0);
+    }
+    return;
+  }
   if (writer) {
     const size_t row = (size_t)item * 64 + islot;
 #pragma unroll
@@ -622,40 +634,38 @@ __device__ __forceinline__ void ens_item(const EnsArgs& a, const typename RecOf<
 
 // P64: items [ib, ie) all have lane shape 64; otherwise 16, 4 or 1 (a switch
 // whose larger register needs stay out of the P = 64 kernel).
-// Waves take queue positions [ib, ie) from a counter (one vector atomic per
-// item, issued by lane 0); the queue lists the longest items first, so the
-// short ones fill the end of the launch.  Every wave leaves once the counter
-// has passed ie.
+// One single-wave work-group per position q of the ordered item list [ib, ie):
+// the list holds the longest items first and the hardware dispatches work-
+// groups in order as wave slots free up, so the short items fill the end of
+// the launch with no queue to contend for.  (A counter that every wave bumped
+// for its next item serialised the small items of large ensembles.)
 template <int ORDER, bool F64, bool NGB, bool P64>
-__global__ __launch_bounds__(256) void k_force_ens(EnsArgs a, const typename RecOf<ORDER, F64>::T* __restrict__ rec,
-                                                   int ib, int ie, int part) {
-  const int lane = threadIdx.x & 63;
-  for (;;) {
-    int k = 0;
-    if (lane == 0) k = atomicAdd(a.ctr + part, 1);
-    const int q = ib + __builtin_amdgcn_readfirstlane(k);
-    if (q >= ie) break;
-    const int item = __builtin_amdgcn_readfirstlane(a.order[q]);
-    // the item is wave-uniform; readfirstlane keeps its fields (and so the
-    // j-record addresses) in SGPRs, which a load the compiler cannot prove
-    // invariant would otherwise place in VGPRs
-    const EnsItem ld = a.items[item];
-    EnsItem it;
-    it.i0 = __builtin_amdgcn_readfirstlane(ld.i0);
-    it.icount = __builtin_amdgcn_readfirstlane(ld.icount);
-    it.jb = __builtin_amdgcn_readfirstlane(ld.jb);
-    it.je = __builtin_amdgcn_readfirstlane(ld.je);
-    it.P = __builtin_amdgcn_readfirstlane(ld.P);
-    it.cluster = __builtin_amdgcn_readfirstlane(ld.cluster);
-    it.pad0 = it.pad1 = 0;
-    if constexpr (P64) {
-      ens_item<ORDER, F64, 64, NGB>(a, rec, it, item);
-    } else {
-      switch (it.P) {
-        case 16: ens_item<ORDER, F64, 16, NGB>(a, rec, it, item); break;
-        case 4: ens_item<ORDER, F64, 4, NGB>(a, rec, it, item); break;
-        default: ens_item<ORDER, F64, 1, NGB>(a, rec, it, item); break;
-      }
+__global__ __launch_bounds__(64) void k_force_ens(EnsArgs a, FinalizeArgs fz,
+                                                  const typename RecOf<ORDER, F64>::T* __restrict__ rec, int ib,
+                                                  int ie) {
+  const int q = ib + (int)blockIdx.x;
+  if (q >= ie) return;
+  const int item = __builtin_amdgcn_readfirstlane(a.order[q]);
+  // the item is wave-uniform; readfirstlane keeps its fields (and so the
+  // j-record addresses) in SGPRs, which a load the compiler cannot prove
+  // invariant would otherwise place in VGPRs
+  const EnsItem ld = a.items[item];
+  EnsItem it;
+  it.i0 = __builtin_amdgcn_readfirstlane(ld.i0);
+  it.icount = __builtin_amdgcn_readfirstlane(ld.icount);
+  it.jb = __builtin_amdgcn_readfirstlane(ld.jb);
+  it.je = __builtin_amdgcn_readfirstlane(ld.je);
+  it.P = __builtin_amdgcn_readfirstlane(ld.P);
+  it.cluster = __builtin_amdgcn_readfirstlane(ld.cluster);
+  it.nsplit = __builtin_amdgcn_readfirstlane(ld.nsplit);
+  it.pad = 0;
+  if constexpr (P64) {
+    ens_item<ORDER, F64, 64, NGB>(a, fz, rec, it, item);
+  } else {
+    switch (it.P) {
+      case 16: ens_item<ORDER, F64, 16, NGB>(a, fz, rec, it, item); break;
+      case 4: ens_item<ORDER, F64, 4, NGB>(a, fz, rec, it, item); break;
+      default: ens_item<ORDER, F64, 1, NGB>(a, fz, rec, it, item); break;
     }
   }
 }
@@ -667,6 +677,7 @@ __global__ __launch_bounds__(256) void k_finalize_ens(FinalizeArgs f, EnsArgs a)
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= f.cd->ni) return;
   const int4 d = a.idesc[i];  // first item, number of splits, i-slot
+  if (!f.ngb_on && d.y == 1) return;  // its only item wrote the row
   double T[NF];
 #pragma unroll
   for (int q = 0; q < NF; ++q) T[q] = 0.0;
@@ -1034,27 +1045,24 @@ int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a) {
   return hip_status();
 }
 
-int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64,
-                     long max_waves) {
+int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64) {
   if (a.nitems <= 0 || f.grid_ni <= 0) return 0;
   const int nsmall = a.nitems - n64;
-  const int nb64 = blocks_for(std::min<long>(n64, max_waves), 4);
-  const int nbs = blocks_for(std::min<long>(nsmall, max_waves), 4);
   dispatch_op(order, f64, [&](auto OR, auto FP) {
     constexpr int O = decltype(OR)::value;
     constexpr bool F = decltype(FP)::value;
     const auto* rec = reinterpret_cast<const typename RecOf<O, F>::T*>(a.rec);
     if (n64 > 0) {
       if (a.ngb_on)
-        hipLaunchKernelGGL((k_force_ens<O, F, true, true>), dim3(nb64), dim3(256), 0, s, a, rec, 0, n64, 0);
+        hipLaunchKernelGGL((k_force_ens<O, F, true, true>), dim3(n64), dim3(64), 0, s, a, f, rec, 0, n64);
       else
-        hipLaunchKernelGGL((k_force_ens<O, F, false, true>), dim3(nb64), dim3(256), 0, s, a, rec, 0, n64, 0);
+        hipLaunchKernelGGL((k_force_ens<O, F, false, true>), dim3(n64), dim3(64), 0, s, a, f, rec, 0, n64);
     }
     if (nsmall > 0) {
       if (a.ngb_on)
-        hipLaunchKernelGGL((k_force_ens<O, F, true, false>), dim3(nbs), dim3(256), 0, s, a, rec, n64, a.nitems, 1);
+        hipLaunchKernelGGL((k_force_ens<O, F, true, false>), dim3(nsmall), dim3(64), 0, s, a, f, rec, n64, a.nitems);
       else
-        hipLaunchKernelGGL((k_force_ens<O, F, false, false>), dim3(nbs), dim3(256), 0, s, a, rec, n64, a.nitems, 1);
+        hipLaunchKernelGGL((k_force_ens<O, F, false, false>), dim3(nsmall), dim3(64), 0, s, a, f, rec, n64, a.nitems);
     }
     hipLaunchKernelGGL((k_finalize_ens<O, F>), dim3(blocks_for(f.grid_ni, 256)), dim3(256), 0, s, f, a);
     if (f.ngb_on) hipLaunchKernelGGL(k_sort_lists, dim3(blocks_for(f.grid_ni, 4)), dim3(256), 0, s, f);

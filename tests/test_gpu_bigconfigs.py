@@ -152,6 +152,13 @@ def test_ensemble_512_clusters_full_size(prec):
         r2 = eng.get_grav_results(1, ni, out=junk)
         for k in ("acc", "jrk", "pot", "nnb", "dsmin"):
             assert np.array_equal(r2[k], r[k], equal_nan=k == "dsmin"), f"out= rows differ: {k}"
+        # without radii (the single-split items write their own result rows): the same rows
+        eng.start_grav_calc_clusters(cidx, I, X, V, eps)
+        r3 = eng.get_grav_results(1, ni)
+        for k in ("acc", "jrk", "pot", "nnb", "dsmin"):
+            assert np.array_equal(r3[k], r[k], equal_nan=k == "dsmin"), f"no-radii rows differ: {k}"
+        eng.start_grav_calc_clusters(cidx, I, X, V, eps, H)
+        r = eng.get_grav_results(1, ni)
         eng.read_ngb_list()
         starts = np.concatenate([[0], np.cumsum([len(p) for p in pick])[:-1]])
         got = {k: [] for k in ("acc", "jrk", "pot")}

@@ -761,9 +761,24 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   if (ni == 0) return 0;
   if (ni < 0 || ni > ENS_MAX_I || !cluster || !eps2c) return fail("start_grav_calc_clusters: bad arguments");
   const int nc = (int)cl_off_.size();
-  for (int i = 0; i < ni; ++i) {
-    if (cluster[i] < 0 || cluster[i] >= nc) return fail("start_grav_calc_clusters: cluster index out of range");
-    if (i && cluster[i] < cluster[i - 1]) return fail("start_grav_calc_clusters: i-particles not grouped by cluster");
+  // the i-particles of each active cluster, validated in the same pass: a run
+  // of equal cluster indices, runs in increasing cluster order
+  struct Grp { int c, ib, n, P, G, S, first; };
+  std::vector<Grp> grps;
+  long work = 0;
+  for (int i = 0; i < ni;) {
+    const int c = cluster[i];
+    if (c < 0 || c >= nc) return fail("start_grav_calc_clusters: cluster index out of range");
+    if (!grps.empty() && c <= grps.back().c)
+      return fail("start_grav_calc_clusters: i-particles not grouped by cluster");
+    int e = i + 1;
+    while (e < ni && cluster[e] == c) ++e;
+    const int n = e - i;
+    const int P = n > 32 ? 64 : n > 8 ? 16 : n > 2 ? 4 : 1;
+    const int G = (n + P - 1) / P;
+    grps.push_back(Grp{c, i, n, P, G, 0, 0});
+    work += (long)G * cl_n_[c];
+    i = e;
   }
   HIPCHK(hipSetDevice(device_), "hipSetDevice");
   if (inflight_ && wait_results() != 0) return -1;
@@ -788,54 +803,67 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
 
   // work table: per active cluster, i-groups of P (by its active count) x j-splits
-  struct Grp { int c, ib, n, P, G; };
-  std::vector<Grp> grps;
-  long work = 0;
-  for (int i = 0; i < ni;) {
-    int e = i;
-    while (e < ni && cluster[e] == cluster[i]) ++e;
-    const int n = e - i;
-    const int P = n > 32 ? 64 : n > 8 ? 16 : n > 2 ? 4 : 1;
-    const int G = (n + P - 1) / P;
-    grps.push_back(Grp{cluster[i], i, n, P, G});
-    work += (long)G * cl_n_[cluster[i]];
-    i = e;
-  }
   const long target = (long)cus_ * 4 * WAVES_PER_SIMD_TARGET;
-  long jch = std::max<long>(MIN_J_PER_SPLIT, (work + target - 1) / target);
-  jch = (jch + JTILE * 16 - 1) / (JTILE * 16) * (JTILE * 16);  // whole 64-interaction folds for Q <= 16
+  const long jraw = std::max<long>(MIN_J_PER_SPLIT, (work + target - 1) / target);
+  // j per split in whole 64-interaction folds of a lane: JTILE for lane shape
+  // P = 64 (one j-lane per i-particle), 16 tiles for the smaller shapes (Q <= 16)
+  const long jch64 = (jraw + JTILE - 1) / JTILE * JTILE;
+  const long jchs = (jraw + JTILE * 16 - 1) / (JTILE * 16) * (JTILE * 16);
   // items of lane shape P = 64 first, the smaller shapes after them: each part
   // runs in a kernel of its own, so the P = 64 part keeps its register budget
-  ens_items_.clear();
-  std::vector<EnsItem> small;
-  ens_idesc_.assign(ni, int4{0, 0, 0, 0});
+  int n64 = 0, nsm = 0;
+  for (Grp& g : grps) {
+    const long jch = g.P == 64 ? jch64 : jchs;
+    g.S = std::max(1, (int)((cl_n_[g.c] + jch - 1) / jch));
+    (g.P == 64 ? n64 : nsm) += g.G * g.S;
+  }
+  int b64 = 0, bsm = n64;
+  for (Grp& g : grps) {
+    int& b = g.P == 64 ? b64 : bsm;
+    g.first = b;
+    b += g.G * g.S;
+  }
+  ens_items_.resize((size_t)n64 + nsm);
   for (const Grp& g : grps) {
     const int off = cl_off_[g.c], nj = cl_n_[g.c];
-    const int S = std::max(1, (int)((nj + jch - 1) / jch));
-    std::vector<EnsItem>& dst = g.P == 64 ? ens_items_ : small;
+    const long jch = g.P == 64 ? jch64 : jchs;
+    EnsItem* it = ens_items_.data() + g.first;
     for (int q = 0; q < g.G; ++q) {
-      const int first = (int)dst.size();
       const int i0 = g.ib + q * g.P;
       const int cnt = std::min(g.P, g.n - q * g.P);
-      for (int s = 0; s < S; ++s) {
+      for (int s = 0; s < g.S; ++s) {
         const int jb = off + (int)std::min<long>((long)s * jch, nj);
         const int je = off + (int)std::min<long>((long)(s + 1) * jch, nj);
-        dst.push_back(EnsItem{i0, cnt, jb, je, g.P, g.c, S, 0});
+        *it++ = EnsItem{i0, cnt, jb, je, g.P, g.c, g.S, 0};
       }
-      for (int k = 0; k < cnt; ++k) ens_idesc_[i0 + k] = int4{first, S, k, g.P == 64 ? 0 : 1};
     }
   }
-  const int n64 = (int)ens_items_.size();
-  ens_items_.insert(ens_items_.end(), small.begin(), small.end());
-  for (int i = 0; i < ni; ++i)
-    if (ens_idesc_[i].w) ens_idesc_[i].x += n64;
+  // per i-particle: its group's first item, the split count, its slot, the part
+  if (ens_idesc_.size() < (size_t)ni) ens_idesc_.resize(ni);
+  par_for((size_t)ni, [&](size_t b, size_t e) {
+    size_t k = std::upper_bound(grps.begin(), grps.end(), (int)b, [](int i, const Grp& g) { return i < g.ib; }) -
+                grps.begin() - 1;
+    for (size_t i = b; i < e; ++i) {
+      while ((int)i >= grps[k].ib + grps[k].n) ++k;
+      const Grp& g = grps[k];
+      const int r = (int)i - g.ib;
+      ens_idesc_[i] = int4{g.first + (r / g.P) * g.S, g.S, r % g.P, g.P == 64 ? 0 : 1};
+    }
+  });
   const int nitems = (int)ens_items_.size();
-  // queue order: longest j-range first within each part
+  // launch order: longest j-range first within each part, equal lengths in
+  // table order (a stable counting sort on the length)
   ens_order_.resize(nitems);
-  for (int k = 0; k < nitems; ++k) ens_order_[k] = k;
-  auto longer = [&](int u, int w) { return ens_items_[u].je - ens_items_[u].jb > ens_items_[w].je - ens_items_[w].jb; };
-  std::stable_sort(ens_order_.begin(), ens_order_.begin() + n64, longer);
-  std::stable_sort(ens_order_.begin() + n64, ens_order_.end(), longer);
+  auto order_part = [&](int k0, int k1) {
+    int maxlen = 0;
+    for (int k = k0; k < k1; ++k) maxlen = std::max(maxlen, ens_items_[k].je - ens_items_[k].jb);
+    std::vector<int> start(maxlen + 2, 0);
+    for (int k = k0; k < k1; ++k) ++start[maxlen - (ens_items_[k].je - ens_items_[k].jb) + 1];
+    for (int l = 1; l <= maxlen + 1; ++l) start[l] += start[l - 1];
+    for (int k = k0; k < k1; ++k) ens_order_[k0 + start[maxlen - (ens_items_[k].je - ens_items_[k].jb)]++] = k;
+  };
+  order_part(0, n64);
+  order_part(n64, nitems);
   if (ens_items_cap_ < nitems) {
     dfree(ens_items_d_);
     dfree(ens_order_d_);
@@ -907,7 +935,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ngb_on_last_ = ngb_on;
   ni_last_ = ni;
   nj_last_ = cl_total_;
-  last_cfg_ = ForceConfig{0, nitems, (int)jch, 0, 0};
+  last_cfg_ = ForceConfig{0, nitems, (int)jch64, 0, 0};
   jhost_valid_ = false;
   return 0;
 }

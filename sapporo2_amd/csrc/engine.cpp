@@ -232,8 +232,8 @@ int Engine::close() {
   slot_lists_h_.clear();
   dfree(gather_d_); gather_cap_ = 0;
   dfree(gaddr_d_); gaddr_cap_ = 0;
-  dfree(idesc_d_); dfree(ens_items_d_); dfree(ens_eps_d_); ens_items_cap_ = ens_eps_cap_ = 0;
-  dfree(ens_order_d_); ens_order_cap_ = 0;
+  dfree(idesc_d_); dfree(ens_eps_d_); ens_eps_cap_ = 0;
+  hfree(ens_tab_h_); dfree(ens_tab_d_); ens_tab_cap_ = 0;
   icap_ = 0; cl_off_.clear(); cl_n_.clear(); cl_total_ = 0;
   for (hipEvent_t* e : {&ev_done_, &ev_d0_, &ev_d1_, &ev_f0_, &ev_f1_, &ev_p0_, &ev_p1_, &ev_s0_, &ev_s1_}) {
     if (*e) (void)hipEventDestroy(*e);
@@ -763,21 +763,20 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   const int nc = (int)cl_off_.size();
   // the i-particles of each active cluster, validated in the same pass: a run
   // of equal cluster indices, runs in increasing cluster order
-  struct Grp { int c, ib, n, P, G, S, first; };
-  std::vector<Grp> grps;
+  std::vector<EnsGroup> grps;
+  std::vector<int> gcl;  // the cluster of each group
   long work = 0;
   for (int i = 0; i < ni;) {
     const int c = cluster[i];
     if (c < 0 || c >= nc) return fail("start_grav_calc_clusters: cluster index out of range");
-    if (!grps.empty() && c <= grps.back().c)
-      return fail("start_grav_calc_clusters: i-particles not grouped by cluster");
+    if (!gcl.empty() && c <= gcl.back()) return fail("start_grav_calc_clusters: i-particles not grouped by cluster");
     int e = i + 1;
     while (e < ni && cluster[e] == c) ++e;
     const int n = e - i;
     const int P = n > 32 ? 64 : n > 8 ? 16 : n > 2 ? 4 : 1;
-    const int G = (n + P - 1) / P;
-    grps.push_back(Grp{c, i, n, P, G, 0, 0});
-    work += (long)G * cl_n_[c];
+    grps.push_back(EnsGroup{i, n, P, 0, 0, 0});
+    gcl.push_back(c);
+    work += (long)((n + P - 1) / P) * cl_n_[c];
     i = e;
   }
   HIPCHK(hipSetDevice(device_), "hipSetDevice");
@@ -812,77 +811,68 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   // items of lane shape P = 64 first, the smaller shapes after them: each part
   // runs in a kernel of its own, so the P = 64 part keeps its register budget
   int n64 = 0, nsm = 0;
-  for (Grp& g : grps) {
+  for (size_t q = 0; q < grps.size(); ++q) {
+    EnsGroup& g = grps[q];
     const long jch = g.P == 64 ? jch64 : jchs;
-    g.S = std::max(1, (int)((cl_n_[g.c] + jch - 1) / jch));
-    (g.P == 64 ? n64 : nsm) += g.G * g.S;
+    g.S = std::max(1, (int)((cl_n_[gcl[q]] + jch - 1) / jch));
+    (g.P == 64 ? n64 : nsm) += (g.n + g.P - 1) / g.P * g.S;
   }
   int b64 = 0, bsm = n64;
-  for (Grp& g : grps) {
+  for (EnsGroup& g : grps) {
     int& b = g.P == 64 ? b64 : bsm;
     g.first = b;
-    b += g.G * g.S;
+    b += (g.n + g.P - 1) / g.P * g.S;
   }
-  ens_items_.resize((size_t)n64 + nsm);
-  for (const Grp& g : grps) {
-    const int off = cl_off_[g.c], nj = cl_n_[g.c];
+  const int nitems = n64 + nsm;
+  const int ng = (int)grps.size();
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t o_order = al(sizeof(EnsItem) * nitems), o_grp = o_order + al(sizeof(int) * nitems);
+  const size_t tab_bytes = o_grp + sizeof(EnsGroup) * ng;
+  if (tab_bytes > ens_tab_cap_) {
+    hfree(ens_tab_h_);
+    dfree(ens_tab_d_);
+    ens_tab_cap_ = std::max(tab_bytes, 2 * ens_tab_cap_);
+    HIPCHK(hmalloc(&ens_tab_h_, ens_tab_cap_), "ensemble work table (host)");
+    HIPCHK(dmalloc(&ens_tab_d_, ens_tab_cap_), "ensemble work table");
+  }
+  EnsItem* items = reinterpret_cast<EnsItem*>(ens_tab_h_);
+  int* order = reinterpret_cast<int*>(ens_tab_h_ + o_order);
+  for (int q = 0; q < ng; ++q) {
+    const EnsGroup& g = grps[q];
+    const int c = gcl[q], off = cl_off_[c], nj = cl_n_[c];
     const long jch = g.P == 64 ? jch64 : jchs;
-    EnsItem* it = ens_items_.data() + g.first;
-    for (int q = 0; q < g.G; ++q) {
-      const int i0 = g.ib + q * g.P;
-      const int cnt = std::min(g.P, g.n - q * g.P);
+    EnsItem* it = items + g.first;
+    for (int i0 = g.ib; i0 < g.ib + g.n; i0 += g.P) {
+      const int cnt = std::min(g.P, g.ib + g.n - i0);
       for (int s = 0; s < g.S; ++s) {
         const int jb = off + (int)std::min<long>((long)s * jch, nj);
         const int je = off + (int)std::min<long>((long)(s + 1) * jch, nj);
-        *it++ = EnsItem{i0, cnt, jb, je, g.P, g.c, g.S, 0};
+        *it++ = EnsItem{i0, cnt, jb, je, g.P, c, g.S, 0};
       }
     }
   }
-  // per i-particle: its group's first item, the split count, its slot, the part
-  if (ens_idesc_.size() < (size_t)ni) ens_idesc_.resize(ni);
-  par_for((size_t)ni, [&](size_t b, size_t e) {
-    size_t k = std::upper_bound(grps.begin(), grps.end(), (int)b, [](int i, const Grp& g) { return i < g.ib; }) -
-                grps.begin() - 1;
-    for (size_t i = b; i < e; ++i) {
-      while ((int)i >= grps[k].ib + grps[k].n) ++k;
-      const Grp& g = grps[k];
-      const int r = (int)i - g.ib;
-      ens_idesc_[i] = int4{g.first + (r / g.P) * g.S, g.S, r % g.P, g.P == 64 ? 0 : 1};
-    }
-  });
-  const int nitems = (int)ens_items_.size();
+  memcpy(ens_tab_h_ + o_grp, grps.data(), sizeof(EnsGroup) * ng);
   // launch order: longest j-range first within each part, equal lengths in
   // table order (a stable counting sort on the length)
-  ens_order_.resize(nitems);
   auto order_part = [&](int k0, int k1) {
     int maxlen = 0;
-    for (int k = k0; k < k1; ++k) maxlen = std::max(maxlen, ens_items_[k].je - ens_items_[k].jb);
+    for (int k = k0; k < k1; ++k) maxlen = std::max(maxlen, items[k].je - items[k].jb);
     std::vector<int> start(maxlen + 2, 0);
-    for (int k = k0; k < k1; ++k) ++start[maxlen - (ens_items_[k].je - ens_items_[k].jb) + 1];
+    for (int k = k0; k < k1; ++k) ++start[maxlen - (items[k].je - items[k].jb) + 1];
     for (int l = 1; l <= maxlen + 1; ++l) start[l] += start[l - 1];
-    for (int k = k0; k < k1; ++k) ens_order_[k0 + start[maxlen - (ens_items_[k].je - ens_items_[k].jb)]++] = k;
+    for (int k = k0; k < k1; ++k) order[k0 + start[maxlen - (items[k].je - items[k].jb)]++] = k;
   };
   order_part(0, n64);
   order_part(n64, nitems);
-  if (ens_items_cap_ < nitems) {
-    dfree(ens_items_d_);
-    dfree(ens_order_d_);
-    ens_items_cap_ = std::max(nitems, 2 * ens_items_cap_);
-    HIPCHK(dmalloc(&ens_items_d_, (size_t)ens_items_cap_), "ensemble work table");
-    HIPCHK(dmalloc(&ens_order_d_, (size_t)ens_items_cap_), "ensemble work queue");
-  }
   if (ens_eps_cap_ < nc) {
     dfree(ens_eps_d_);
     ens_eps_cap_ = nc;
     HIPCHK(dmalloc(&ens_eps_d_, (size_t)nc), "ensemble softening");
   }
-  HIPCHK(hipMemcpyAsync(ens_items_d_, ens_items_.data(), sizeof(EnsItem) * nitems, hipMemcpyHostToDevice, stream_),
-         "work table upload");
-  HIPCHK(hipMemcpyAsync(ens_order_d_, ens_order_.data(), sizeof(int) * nitems, hipMemcpyHostToDevice, stream_),
-         "work queue upload");
-  HIPCHK(hipMemcpyAsync(idesc_d_, ens_idesc_.data(), sizeof(int4) * ni, hipMemcpyHostToDevice, stream_),
-         "descriptor upload");
+  HIPCHK(hipMemcpyAsync(ens_tab_d_, ens_tab_h_, tab_bytes, hipMemcpyHostToDevice, stream_), "work table upload");
   HIPCHK(hipMemcpyAsync(ens_eps_d_, eps2c, sizeof(double) * nc, hipMemcpyHostToDevice, stream_), "eps upload");
+  if (launch_ens_idesc(stream_, reinterpret_cast<const EnsGroup*>(ens_tab_d_ + o_grp), ng, ni, idesc_d_) != 0)
+    return fail("ensemble descriptor launch", hipGetLastError());
 
   int nupd = 0;
   if (upload_updates(&nupd) != 0) return -1;
@@ -905,9 +895,9 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   ea.ih2 = ibuf_d_ + (bh - bx);
   ea.iid = iid_d_;
   ea.eps2 = ens_eps_d_;
-  ea.items = ens_items_d_;
+  ea.items = reinterpret_cast<const EnsItem*>(ens_tab_d_);
   ea.nitems = nitems;
-  ea.order = ens_order_d_;
+  ea.order = reinterpret_cast<const int*>(ens_tab_d_ + o_order);
   ea.idesc = idesc_d_;
   ea.part = part_d_;
   ea.part_ds2 = pds2_d_;

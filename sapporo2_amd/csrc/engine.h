@@ -238,12 +238,10 @@ class Engine {
   int icap_ = 0;
   std::vector<int> cl_off_, cl_n_;
   int cl_total_ = 0;
-  std::vector<sapk::EnsItem> ens_items_;
-  std::vector<int4> ens_idesc_;
-  sapk::EnsItem* ens_items_d_ = nullptr; int ens_items_cap_ = 0;
+  // work table of an ensemble call: [items | launch order | groups], built in
+  // pinned host memory and uploaded with one copy
+  char* ens_tab_h_ = nullptr; char* ens_tab_d_ = nullptr; size_t ens_tab_cap_ = 0;
   int4* idesc_d_ = nullptr;
-  std::vector<int> ens_order_;
-  int* ens_order_d_ = nullptr; int ens_order_cap_ = 0;
   double* ens_eps_d_ = nullptr; int ens_eps_cap_ = 0;
 
   struct TuneEntry { int order, prec, ngb, li, lj, P, jlog; };

@@ -165,6 +165,10 @@ struct FinalizeArgs {
 // nsplit items that cover the cluster's j for these i-particles.
 struct EnsItem { int i0, icount, jb, je, P, cluster, nsplit, pad; };
 
+// The i-particles [ib, ib + n) of one cluster in an ensemble call: lane shape P,
+// S splits per i-group, items first .. first + ceil(n / P) * S - 1.
+struct EnsGroup { int ib, n, P, S, first, pad; };
+
 struct EnsArgs {
   const void* rec; const int2* idr;
   const double* ipos; const double* ivel; const double* iacc; const double* ih2; const int* iid;
@@ -191,6 +195,9 @@ int launch_finalize(hipStream_t s, int order, bool f64, const FinalizeArgs& a);
 // writes its i-particles' result rows itself, and the finalize skips them.
 int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const FinalizeArgs& f, int n64);
 int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a);
+// Per i-particle of an ensemble call (ni of them, in ng groups): {first item of
+// its i-group, split count, slot in the i-group, 1 for the small-shape part}.
+int launch_ens_idesc(hipStream_t s, const EnsGroup* g, int ng, int ni, int4* idesc);
 // Writes the call descriptor v to device memory cd with a one-thread kernel
 // (no copy command, nothing allocated).
 int launch_write_desc(hipStream_t s, CallDesc* cd, const CallDesc& v);

@@ -1070,6 +1070,29 @@ int launch_force_ens(hipStream_t s, int order, bool f64, const EnsArgs& a, const
   return hip_status();
 }
 
+// One thread per i-particle: its group by binary search over the groups' first
+// i-particles (groups are in i order and cover [0, ni)).
+__global__ __launch_bounds__(256) void k_ens_idesc(const EnsGroup* __restrict__ g, int ng, int ni,
+                                                   int4* __restrict__ idesc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ni) return;
+  int lo = 0, hi = ng - 1;  // the last group with ib <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].ib <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const EnsGroup gr = g[lo];
+  const int r = i - gr.ib;
+  idesc[i] = make_int4(gr.first + (r / gr.P) * gr.S, gr.S, r % gr.P, gr.P == 64 ? 0 : 1);
+}
+
+int launch_ens_idesc(hipStream_t s, const EnsGroup* g, int ng, int ni, int4* idesc) {
+  if (ni <= 0 || ng <= 0) return 0;
+  hipLaunchKernelGGL(k_ens_idesc, dim3(blocks_for(ni, 256)), dim3(256), 0, s, g, ng, ni, idesc);
+  return hip_status();
+}
+
 int launch_gather_pred(hipStream_t s, int order, bool f64, const GatherArgs& a) {
   if (a.n <= 0) return 0;
   const int nb = blocks_for(a.n, 256);

@@ -1,15 +1,20 @@
 // engine.cpp -- host engine (see engine.h).  Reference behaviour cited per method.
 #include "engine.h"
+#include "hostpool.h"
 
 #include <dlfcn.h>
 #include <immintrin.h>
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <system_error>
 #include <thread>
 #include <vector>
@@ -41,36 +46,6 @@ constexpr size_t CALL_HDR = 256;
 inline size_t call_ids_bytes(int bucket) { return ((size_t)bucket * sizeof(int) + 255) / 256 * 256; }
 constexpr size_t CALL_DOUBLES = (CALL_HDR + ((size_t)NPIPES * sizeof(int) + 255) / 256 * 256) / sizeof(double) +
                                 (size_t)IBUF_FIELDS * NPIPES;
-
-// Host-side copies of very large calls (ensembles carry millions of
-// i-particles per call) are split over threads: fn(begin, end) on [0, n) in
-// chunks of at least HOST_GRAIN elements, at most HOST_THREADS threads, the
-// calling thread taking the first chunk.  Smaller ranges run inline.
-constexpr size_t HOST_GRAIN = 65536;
-constexpr unsigned HOST_THREADS = 16;
-template <typename F>
-void par_for(size_t n, F&& fn) {
-  const unsigned hw = std::max(1u, std::min(HOST_THREADS, std::thread::hardware_concurrency()));
-  const size_t T = std::min<size_t>(hw, n / HOST_GRAIN);
-  if (T <= 1) {
-    fn((size_t)0, n);
-    return;
-  }
-  const size_t chunk = (n + T - 1) / T;
-  std::vector<std::thread> th;
-  th.reserve(T - 1);
-  for (size_t t = 1; t < T; ++t) {
-    const size_t b = t * chunk, e = std::min(n, b + chunk);
-    if (b >= e) break;
-    try {
-      th.emplace_back([&fn, b, e] { fn(b, e); });
-    } catch (const std::system_error&) {
-      fn(b, e);  // no thread available: this chunk runs here
-    }
-  }
-  fn((size_t)0, std::min(n, chunk));
-  for (std::thread& x : th) x.join();
-}
 
 template <typename T>
 hipError_t dmalloc(T** p, size_t n) {

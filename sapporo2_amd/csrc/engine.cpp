@@ -698,14 +698,17 @@ int Engine::cluster_offset(int c) const {
 // Forces for i-particles of many clusters in one launch.  The i-particles
 // are grouped by cluster (non-decreasing cluster index); each interacts with
 // its own cluster's j-particles only, with that cluster's eps2.
-// Packs the i-particles into the pinned upload buffer: [xi | vi | (ai) | h2]
-// doubles and ids (reference sapporohostclass.cpp:613-649); returns whether
-// any h2 > 0 (neighbour search on).
-bool Engine::pack_i(int ni, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
-                    const double* h2, double* bx, double* bv, double* ba, double* bh, int* bid) {
+// Packs i-particles [i0, i1) into the pinned upload buffer: [xi | vi | (ai) |
+// h2] doubles and ids (reference sapporohostclass.cpp:613-649); returns
+// whether any of them has h2 > 0 (neighbour search on).
+bool Engine::pack_i(int i0, int i1, const int* id, const double (*xi)[3], const double (*vi)[3],
+                    const double (*ai)[3], const double* h2, double* bx, double* bv, double* ba, double* bh,
+                    int* bid) {
   const bool h6 = order_ >= ORDER_H6;
   std::atomic<bool> ngb_on{false};
-  par_for((size_t)ni, [&](size_t b, size_t e) {
+  par_for((size_t)(i1 - i0), [&](size_t b, size_t e) {
+    b += (size_t)i0;
+    e += (size_t)i0;
     const size_t n = e - b;
     memcpy(bx + 3 * b, xi[b], sizeof(double) * 3 * n);
     if (vi) memcpy(bv + 3 * b, vi[b], sizeof(double) * 3 * n);
@@ -771,10 +774,27 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   double* bv = bx + 3 * (size_t)ni;
   double* ba = bv + 3 * (size_t)ni;
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
-  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh, iid_h_);
-  const size_t nd = (size_t)(bh + ni - bx);
-  HIPCHK(hipMemcpyAsync(ibuf_d_, ibuf_h_, nd * sizeof(double), hipMemcpyHostToDevice, stream_), "i upload");
-  HIPCHK(hipMemcpyAsync(iid_d_, iid_h_, (size_t)ni * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
+  // Large calls pack and upload in chunks: each chunk's copies are queued as
+  // soon as it is packed, so the transfer overlaps the packing of the rest.
+  const bool h6 = order_ >= ORDER_H6;
+  const int nchunk = ni >= (1 << 18) ? 4 : ni >= (1 << 16) ? 2 : 1;
+  bool ngb_on = false;
+  for (int k = 0; k < nchunk; ++k) {
+    const int b = (int)((long)ni * k / nchunk), e = (int)((long)ni * (k + 1) / nchunk);
+    ngb_on |= pack_i(b, e, id, xi, vi, ai, h2, bx, bv, ba, bh, iid_h_);
+    const size_t n = (size_t)(e - b);
+    auto up = [&](const double* h) {
+      return hipMemcpyAsync(ibuf_d_ + (h - ibuf_h_), h, n * sizeof(double), hipMemcpyHostToDevice, stream_);
+    };
+    auto up3 = [&](const double* h) {
+      return hipMemcpyAsync(ibuf_d_ + (h - ibuf_h_), h, 3 * n * sizeof(double), hipMemcpyHostToDevice, stream_);
+    };
+    HIPCHK(up3(bx + 3 * (size_t)b), "i upload");
+    HIPCHK(up3(bv + 3 * (size_t)b), "i upload");
+    if (h6) HIPCHK(up3(ba + 3 * (size_t)b), "i upload");
+    HIPCHK(up(bh + b), "i upload");
+    HIPCHK(hipMemcpyAsync(iid_d_ + b, iid_h_ + b, n * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
+  }
 
   // work table: per active cluster, i-groups of P (by its active count) x j-splits
   const long target = (long)cus_ * 4 * WAVES_PER_SIMD_TARGET;
@@ -930,7 +950,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   double* bv = bx + 3 * (size_t)ni;
   double* ba = bv + 3 * (size_t)ni;
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
-  const bool ngb_on = pack_i(ni, id, xi, vi, ai, h2, bx, bv, ba, bh, bid);
+  const bool ngb_on = pack_i(0, ni, id, xi, vi, ai, h2, bx, bv, ba, bh, bid);
   const size_t nd = (size_t)(bh + ni - bx);
   if (ngb_on) {
     if (ensure_lists(NPIPES) != 0) return -1;

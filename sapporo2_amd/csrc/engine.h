@@ -132,7 +132,7 @@ class Engine {
   int scatter_updates(int n, int ni_reset, int pred_j1 = 0);
   int predict_range(int j0, int j1);
   int post_desc(int ni, int nupd, int ni_reset, double eps2);
-  bool pack_i(int ni, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
+  bool pack_i(int i0, int i1, const int* id, const double (*xi)[3], const double (*vi)[3], const double (*ai)[3],
               const double* h2, double* bx, double* bv, double* ba, double* bh, int* bid);
   int scatter_blocks(int nupd, int ni_reset) const;
   int enqueue_call(int grid_ni, int nj, const sapk::ForceConfig& cfg, bool ngb_on, bool predict,

@@ -722,9 +722,8 @@ bool Engine::pack_i(int i0, int i1, const int* id, const double (*xi)[3], const 
       bool any = false;
       for (size_t i = b; i < e; ++i) any |= h2[i] > 0.0;
       if (any) ngb_on.store(true, std::memory_order_relaxed);
-    } else {
-      memset(bh + b, 0, sizeof(double) * n);
-    }
+    }  // without radii the h2 region is neither written nor uploaded: the kernels read it
+       // only with the neighbour search on
     if (id) memcpy(bid + b, id + b, sizeof(int) * n);
     else std::fill(bid + b, bid + e, -1);
   });
@@ -792,7 +791,7 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
     HIPCHK(up3(bx + 3 * (size_t)b), "i upload");
     HIPCHK(up3(bv + 3 * (size_t)b), "i upload");
     if (h6) HIPCHK(up3(ba + 3 * (size_t)b), "i upload");
-    HIPCHK(up(bh + b), "i upload");
+    if (h2) HIPCHK(up(bh + b), "i upload");
     HIPCHK(hipMemcpyAsync(iid_d_ + b, iid_h_ + b, n * sizeof(int), hipMemcpyHostToDevice, stream_), "i upload");
   }
 
@@ -951,7 +950,7 @@ int Engine::start_grav_calc(int nj, int ni, const int* id, const double (*xi)[3]
   double* ba = bv + 3 * (size_t)ni;
   double* bh = ba + (order_ >= ORDER_H6 ? 3 * (size_t)ni : 0);
   const bool ngb_on = pack_i(0, ni, id, xi, vi, ai, h2, bx, bv, ba, bh, bid);
-  const size_t nd = (size_t)(bh + ni - bx);
+  const size_t nd = (size_t)((h2 ? bh + ni : bh) - bx);
   if (ngb_on) {
     if (ensure_lists(NPIPES) != 0) return -1;
     if (ensure_packed((long)NPIPES * NGB_CAP) != 0) return -1;

@@ -15,7 +15,8 @@
 #include <mutex>
 #include <system_error>
 #include <thread>
-#include <vector>
+
+#include <unistd.h>
 
 namespace sap {
 
@@ -35,16 +36,18 @@ constexpr long HOST_SPIN_US = 300;
 
 class HostPool {
  public:
+  // Never destroyed: the workers are detached and run until the process ends.
   static HostPool& get() {
-    static HostPool pool;
-    return pool;
+    static HostPool* pool = new HostPool;
+    return *pool;
   }
-  unsigned threads() const { return (unsigned)th_.size() + 1; }
+  unsigned threads() const { return nworkers_ + 1; }
 
   // fn(begin, end) over [0, n) in chunks of `chunk` elements.  Returns false,
   // having run nothing, while another thread's job holds the pool.
   template <typename F>
   bool run(size_t n, size_t chunk, F& fn) {
+    if (getpid() != pid_) return false;  // a forked child has none of the workers
     std::unique_lock<std::mutex> own(run_m_, std::try_to_lock);
     if (!own.owns_lock()) return false;
     auto j = std::make_shared<Job>();
@@ -92,20 +95,12 @@ class HostPool {
     spin_ = std::chrono::microseconds(spin);
     for (unsigned t = 1; t < want; ++t) {
       try {
-        th_.emplace_back([this] { loop(); });
+        std::thread([this] { loop(); }).detach();
+        ++nworkers_;
       } catch (const std::system_error&) {
         break;  // fewer workers; the caller takes the chunks no worker takes
       }
     }
-  }
-
-  ~HostPool() {
-    {
-      std::lock_guard<std::mutex> lk(m_);
-      stop_.store(true);
-    }
-    cv_.notify_all();
-    for (std::thread& t : th_) t.join();
   }
 
   void loop() {
@@ -113,11 +108,10 @@ class HostPool {
     for (;;) {
       const auto t0 = std::chrono::steady_clock::now();
       for (unsigned k = 1;; ++k) {
-        if (stop_.load(std::memory_order_relaxed)) return;
         if (gen_.load(std::memory_order_acquire) != seen) break;
         if ((k & 63) == 0 && std::chrono::steady_clock::now() - t0 > spin_) {
           std::unique_lock<std::mutex> lk(m_);
-          cv_.wait(lk, [&] { return stop_.load() || gen_.load() != seen; });
+          cv_.wait(lk, [&] { return gen_.load() != seen; });
         } else {
           _mm_pause();
         }
@@ -128,12 +122,12 @@ class HostPool {
     }
   }
 
-  std::vector<std::thread> th_;
+  unsigned nworkers_ = 0;
+  const pid_t pid_ = getpid();
   std::mutex run_m_;  // one job at a time
   std::mutex m_;      // sleeping workers
   std::condition_variable cv_;
   std::atomic<uint64_t> gen_{0};
-  std::atomic<bool> stop_{false};
   std::shared_ptr<Job> job_;
   std::chrono::microseconds spin_{HOST_SPIN_US};
 };

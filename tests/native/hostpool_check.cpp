@@ -1,9 +1,14 @@
 // Checks the engine's host thread pool (sapporo2_amd/csrc/hostpool.h) on the
 // CPU: every element of every job is visited exactly once, jobs of all sizes
 // (inline and pooled), two caller threads at once (one of them runs inline
-// while the other holds the pool), and workers that sleep between jobs.
+// while the other holds the pool), workers that sleep between jobs, and a
+// forked child (which has none of the workers and runs its jobs itself).
+#include <sys/wait.h>
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -35,6 +40,15 @@ int main(int argc, char** argv) {
   std::thread b([&] { bad3 = check(11, reps); });
   a.join();
   b.join();
-  std::printf("threads %u: %s\n", sap::HostPool::get().threads(), bad | bad2 | bad3 ? "FAIL" : "ok");
-  return bad | bad2 | bad3;
+  int bad4 = 0;
+  if (argc > 2 && std::strcmp(argv[2], "fork") == 0) {
+    const pid_t p = fork();
+    if (p == 0) _exit(check(13, 20));
+    int st = 0;
+    bad4 = (p < 0 || waitpid(p, &st, 0) != p || !WIFEXITED(st) || WEXITSTATUS(st) != 0) ? 1 : 0;
+    bad4 |= check(17, 20);  // the parent's pool still works
+  }
+  const int bad_all = bad | bad2 | bad3 | bad4;
+  std::printf("threads %u: %s\n", sap::HostPool::get().threads(), bad_all ? "FAIL" : "ok");
+  return bad_all;
 }

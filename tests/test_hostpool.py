@@ -1,6 +1,7 @@
 """The host thread pool that packs and unpacks large calls (sapporo2_amd/csrc/hostpool.h),
 built on the CPU with ThreadSanitizer: every element of every job is visited exactly once,
-with spinning and with sleeping workers, and with two caller threads at once."""
+with spinning and with sleeping workers, with two caller threads at once, and in a forked
+child."""
 from __future__ import annotations
 
 import os
@@ -27,10 +28,11 @@ def harness():
     return exe
 
 
-@pytest.mark.parametrize("threads,spin", [("4", "300"), ("4", "0"), ("1", "300"), ("8", "50")])
-def test_hostpool_visits_each_element_once(harness, threads, spin):
+@pytest.mark.parametrize("threads,spin,mode", [("4", "300", ""), ("4", "0", ""), ("1", "300", ""),
+                                               ("8", "50", ""), ("4", "300", "fork")])
+def test_hostpool_visits_each_element_once(harness, threads, spin, mode):
     env = dict(os.environ, SAPPORO2_AMD_HOST_THREADS=threads, SAPPORO2_AMD_HOST_SPIN_US=spin,
                TSAN_OPTIONS="halt_on_error=1")
-    r = subprocess.run([harness, "150"], capture_output=True, text=True, env=env, timeout=300)
+    r = subprocess.run([harness, "150", mode], capture_output=True, text=True, env=env, timeout=300)
     assert r.returncode == 0, (r.stdout + r.stderr)[-4000:]
     assert "ok" in r.stdout and "WARNING: ThreadSanitizer" not in r.stderr, r.stderr[-4000:]

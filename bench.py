@@ -454,7 +454,8 @@ def extra_ensemble(nthreads, cpu_budget):
                 dev = 0.0
                 for k, (cidx, I, X, V, H, _) in enumerate(calls):
                     eng.start_grav_calc_clusters(cidx, I, X, V, eps)
-                    r = eng.get_grav_results(1, len(I), ngb=True, out=buf)
+                    # a compiled caller's arrays: dsmin untouched where no neighbour is found
+                    r = eng.get_grav_results(1, len(I), ngb=True, out=buf, keep_dsmin=buf is not None)
                     if buf is None:
                         buf = r      # the first call is the largest (all clusters fully active)
                     dev += eng.last_device_ms()

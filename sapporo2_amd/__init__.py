@@ -230,10 +230,13 @@ class Engine:
             self._h, int(nj), ni, _pi(ids_), _pd(xi_), _pd(vi_), _pd(ai_), float(eps2), _pd(h2_)),
             "start_grav_calc")
 
-    def get_grav_results(self, nj, ni, ngb=True, out=None):
+    def get_grav_results(self, nj, ni, ngb=True, out=None, keep_dsmin=False):
         """Returns dict acc, jrk, snp (SIXTH), pot, nnb, dsmin (NaN where not written).
         out: a dict of that form from an earlier call, with arrays of at least ni rows,
-        whose first ni rows are filled and returned instead of new arrays."""
+        whose first ni rows are filled and returned instead of new arrays.  keep_dsmin
+        (with out): dsmin keeps the caller's values where no nearest neighbour was found,
+        as the reference's getGravResults leaves them (sapporohostclass.cpp:705-802),
+        instead of being refilled with NaN first."""
         if out is None:
             acc = np.zeros((ni, 3)); jrk = np.zeros((ni, 3)); pot = np.zeros(ni)
             snp = np.zeros((ni, 3)) if self.order >= SIXTH else None
@@ -251,7 +254,8 @@ class Engine:
             snp = rows("snp", (3,), np.float64) if self.order >= SIXTH else None
             crk = rows("crk", (3,), np.float64) if self.order >= SIXTH else None
             nnb = rows("nnb", (), np.int32); dsmin = rows("dsmin", (), np.float64)
-            dsmin.fill(np.nan)
+            if not keep_dsmin:
+                dsmin.fill(np.nan)
             if not ngb:
                 nnb.fill(0)
         self._check(self._L.sapamd_get_grav_results(

@@ -152,6 +152,14 @@ def test_ensemble_512_clusters_full_size(prec):
         r2 = eng.get_grav_results(1, ni, out=junk)
         for k in ("acc", "jrk", "pot", "nnb", "dsmin"):
             assert np.array_equal(r2[k], r[k], equal_nan=k == "dsmin"), f"out= rows differ: {k}"
+        # keep_dsmin: dsmin written where a nearest neighbour was found, the caller's
+        # values (here 5.0) kept elsewhere, as the reference leaves them
+        junk["dsmin"].fill(5.0)
+        r2 = eng.get_grav_results(1, ni, out=junk, keep_dsmin=True)
+        found = ~np.isnan(r["dsmin"])
+        assert np.array_equal(r2["dsmin"][found], r["dsmin"][found]), "keep_dsmin: written rows"
+        assert np.all(r2["dsmin"][~found] == 5.0) and np.all(junk["dsmin"][ni:] == 5.0), "keep_dsmin: kept rows"
+        assert np.array_equal(r2["nnb"], r["nnb"]), "keep_dsmin: nnb"
         # without radii (the single-split items write their own result rows): the same rows
         eng.start_grav_calc_clusters(cidx, I, X, V, eps)
         r3 = eng.get_grav_results(1, ni)

@@ -738,23 +738,59 @@ int Engine::start_grav_calc_clusters(int ni, const int* cluster, const int* id, 
   if (ni == 0) return 0;
   if (ni < 0 || ni > ENS_MAX_I || !cluster || !eps2c) return fail("start_grav_calc_clusters: bad arguments");
   const int nc = (int)cl_off_.size();
-  // the i-particles of each active cluster, validated in the same pass: a run
-  // of equal cluster indices, runs in increasing cluster order
+  // The i-particles of each active cluster: a run of equal cluster indices,
+  // runs in increasing cluster order.  The run starts are found in parallel;
+  // the groups are then built from them in order.  Invalid input is
+  // diagnosed by the serial scan below, which names the first fault.
+  std::vector<int> starts;
+  {
+    std::mutex mu;
+    std::vector<std::pair<size_t, std::vector<int>>> found;
+    std::atomic<bool> bad{false};
+    par_for((size_t)ni, [&](size_t b, size_t e) {
+      std::vector<int> bnd;
+      bool fault = false;
+      for (size_t i = b; i < e; ++i) {
+        const int c = cluster[i];
+        fault |= c < 0 || c >= nc;
+        if (i > 0 && cluster[i - 1] != c) {
+          fault |= cluster[i - 1] > c;
+          bnd.push_back((int)i);
+        }
+      }
+      if (fault) bad.store(true, std::memory_order_relaxed);
+      std::lock_guard<std::mutex> lk(mu);
+      found.emplace_back(b, std::move(bnd));
+    });
+    if (bad.load()) {
+      for (int i = 0, prev = -1; i < ni; ++i) {
+        const int c = cluster[i];
+        if (c < 0 || c >= nc) return fail("start_grav_calc_clusters: cluster index out of range");
+        if (i > 0 && c != cluster[i - 1] && c <= prev)
+          return fail("start_grav_calc_clusters: i-particles not grouped by cluster");
+        prev = c;
+      }
+    }
+    std::sort(found.begin(), found.end(),
+              [](const std::pair<size_t, std::vector<int>>& x, const std::pair<size_t, std::vector<int>>& y) {
+                return x.first < y.first;
+              });
+    starts.push_back(0);
+    for (const auto& f : found) starts.insert(starts.end(), f.second.begin(), f.second.end());
+  }
   std::vector<EnsGroup> grps;
   std::vector<int> gcl;  // the cluster of each group
+  grps.reserve(starts.size());
+  gcl.reserve(starts.size());
   long work = 0;
-  for (int i = 0; i < ni;) {
+  for (size_t q = 0; q < starts.size(); ++q) {
+    const int i = starts[q], e = q + 1 < starts.size() ? starts[q + 1] : ni;
     const int c = cluster[i];
-    if (c < 0 || c >= nc) return fail("start_grav_calc_clusters: cluster index out of range");
-    if (!gcl.empty() && c <= gcl.back()) return fail("start_grav_calc_clusters: i-particles not grouped by cluster");
-    int e = i + 1;
-    while (e < ni && cluster[e] == c) ++e;
     const int n = e - i;
     const int P = n > 32 ? 64 : n > 8 ? 16 : n > 2 ? 4 : 1;
     grps.push_back(EnsGroup{i, n, P, 0, 0, 0});
     gcl.push_back(c);
     work += (long)((n + P - 1) / P) * cl_n_[c];
-    i = e;
   }
   HIPCHK(hipSetDevice(device_), "hipSetDevice");
   if (inflight_ && wait_results() != 0) return -1;

@@ -67,3 +67,41 @@ def test_driver_interleaved_steps_and_parity():
         if c != nn["ngb_cnt"][q] or (c < 256 and not np.array_equal(got, nn["ngb_list"][q, :c])):
             bad += 1
     assert bad == 0, f"neighbour-list mismatches {bad}"
+
+
+def test_driver_host_threads_do_not_change_results():
+    """The host copies of each call (pack, unpack) on one thread or on the pool: the same
+    bytes in every result and predicted-state field of the driver's output and in every
+    neighbour list below the 256 cap.  (A list at the cap holds whichever 256 neighbours
+    the appends stored first, in any run, as the reference's does.)"""
+    print("g6_blocksteps: Plummer 8192, host copies on 1 thread vs the pool", flush=True)
+    w = bench.Workload(8192, 32)
+    tmp = tempfile.mkdtemp()
+    wl = os.path.join(tmp, "w.wl")
+    bench.write_workload(wl, w.n, w.ids, w.m, w.x, w.v, w.acc, w.jrk, w.tj, w.h2, w.px, w.pv, w.blocks,
+                         bench.EPS2, True, True)
+    res = {}
+    old = os.environ.get("SAPPORO2_AMD_HOST_THREADS")
+    try:
+        for t in ("1", "8"):
+            os.environ["SAPPORO2_AMD_HOST_THREADS"] = t
+            rp = os.path.join(tmp, f"t{t}.res")
+            bench.run_driver(wl, rp, "mixed", 1, 0, len(w.blocks) - 1, timing=0)
+            res[t] = bench.read_results(rp)
+    finally:
+        if old is None:
+            os.environ.pop("SAPPORO2_AMD_HOST_THREADS", None)
+        else:
+            os.environ["SAPPORO2_AMD_HOST_THREADS"] = old
+    for k in res["1"]:
+        if k != "list":
+            assert np.array_equal(res["1"][k], res["8"][k]), f"field {k} differs with the host pool"
+    cnt = res["1"]["cnt"]
+    stored = np.minimum(cnt, 256)
+    starts = np.concatenate([[0], np.cumsum(stored)[:-1]])
+    below = np.flatnonzero(cnt < 256)
+    assert len(below) > len(cnt) // 2
+    for i in below:
+        a, b = res["1"]["list"], res["8"]["list"]
+        s = starts[i]
+        assert np.array_equal(a[s:s + cnt[i]], b[s:s + cnt[i]]), f"list of i-particle {i} differs"
